@@ -1,0 +1,139 @@
+"""Device-side B2SR matrices and BMV operators (torch tensors as device memory).
+
+Torch only allocates device memory and supplies the current HIP stream; every
+computation runs in libbgb's HIP kernels through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import call, lib
+
+# Raw storage dtypes for the B2SR words (bit patterns; view as unsigned on host).
+WORD_TORCH = {4: torch.uint8, 8: torch.uint8, 16: torch.int16, 32: torch.int32}
+WORD_NUMPY = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32}
+TILE_DIMS = (4, 8, 16, 32)
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t):
+    return t.data_ptr() if t is not None else 0
+
+
+def packed_vec_words(T, nblockrows):
+    return int(lib().bgb_packed_vec_words(T, nblockrows))
+
+
+class B2SR:
+    """Bit-packed block sparse row matrix with tile dim T (the reference's layout)."""
+
+    def __init__(self, T, nrows, ncols, rowptr, colind, val, nblocks):
+        assert T in TILE_DIMS
+        self.T, self.nrows, self.ncols = T, int(nrows), int(ncols)
+        self.rowptr, self.colind, self.val = rowptr, colind, val
+        self.nblocks = int(nblocks)
+
+    @property
+    def nblockrows(self):
+        return (self.nrows + self.T - 1) // self.T
+
+    @property
+    def device(self):
+        return self.rowptr.device
+
+    def nbytes(self):
+        """Bytes of rowptr + colind + tile words."""
+        wb = lib().bgb_word_bytes(self.T)
+        return 4 * (self.nblockrows + 1) + 4 * self.nblocks + self.nblocks * self.T * wb
+
+    @classmethod
+    def from_csr(cls, csr, T, device="cuda"):
+        """Packs a host CSR (bgb.graph.HostCSR) on the GPU: nnzb pass then fill."""
+        dev = torch.device(device)
+        d_rp = torch.from_numpy(csr.rowptr).to(dev)
+        d_ci = torch.from_numpy(csr.colind if csr.nnz else np.zeros(1, np.int32)).to(dev)
+        return cls.from_device_csr(T, csr.nrows, csr.ncols, csr.nnz, d_rp, d_ci)
+
+    @classmethod
+    def from_device_csr(cls, T, nrows, ncols, nnz, d_rowptr, d_colind):
+        dev = d_rowptr.device
+        nbr = (nrows + T - 1) // T
+        bsr_rp = torch.empty(nbr + 1, dtype=torch.int32, device=dev)
+        nb = ctypes.c_int(0)
+        call("bgb_b2sr_nnzb", T, nrows, ncols, nnz, _ptr(d_rowptr), _ptr(d_colind), _ptr(bsr_rp),
+             ctypes.byref(nb), _stream())
+        nblocks = nb.value
+        # 16-byte padding keeps vector loads of the last tiles inside the allocation
+        colind = torch.empty(max(nblocks, 1) + 4, dtype=torch.int32, device=dev)
+        val = torch.zeros(max(nblocks, 1) * T + 16, dtype=WORD_TORCH[T], device=dev)
+        call("bgb_csr2b2sr", T, nrows, ncols, nnz, _ptr(d_rowptr), _ptr(d_colind), _ptr(bsr_rp),
+             nblocks, _ptr(colind), _ptr(val), _stream())
+        return cls(T, nrows, ncols, bsr_rp, colind, val, nblocks)
+
+    def to_host(self):
+        """(rowptr, colind, val) as numpy arrays (val as unsigned words)."""
+        rp = self.rowptr.cpu().numpy()
+        ci = self.colind[: self.nblocks].cpu().numpy()
+        v = self.val[: self.nblocks * self.T].cpu().numpy().view(WORD_NUMPY[self.T])
+        return rp, ci, v
+
+
+def pack_vec(x, T, nblockrows=None):
+    """ToBit{T}Row: float vector -> packed words (length ceil(NBR/T)*T)."""
+    assert x.dtype == torch.float32 and x.is_cuda
+    n = x.numel()
+    nbr = (n + T - 1) // T if nblockrows is None else nblockrows
+    words = packed_vec_words(T, nbr)
+    xb = torch.zeros(max(words, 1), dtype=WORD_TORCH[T], device=x.device)
+    call("bgb_pack_vec", T, _ptr(x), n, _ptr(xb), _stream())
+    return xb
+
+
+def _out(out, n, dtype, device, accumulate):
+    if out is None:
+        return torch.zeros(n, dtype=dtype, device=device)
+    assert out.numel() >= n and out.dtype == dtype
+    return out
+
+
+def bmv_bin_bin_bin(A, xb, out=None, accumulate=False):
+    C = _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, accumulate)
+    call("bgb_bmv_bin_bin_bin", A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr), _ptr(A.colind),
+         A.nblockrows, A.nblocks, int(accumulate), _stream())
+    return C
+
+
+def bmv_bin_bin_full(A, xb, out=None, accumulate=False, dtype=torch.int32):
+    C = _out(out, A.nblockrows * A.T, dtype, A.device, accumulate)
+    fn = "bgb_bmv_bin_bin_full_i32" if dtype == torch.int32 else "bgb_bmv_bin_bin_full_f32"
+    call(fn, A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows,
+         A.nblocks, int(accumulate), _stream())
+    return C
+
+
+def bmv_bin_full_full(A, x, out=None, accumulate=False):
+    assert x.dtype == torch.float32 and x.numel() >= A.nblockrows * A.T
+    C = _out(out, A.nblockrows * A.T, torch.float32, A.device, accumulate)
+    call("bgb_bmv_bin_full_full", A.T, _ptr(A.val), _ptr(x), _ptr(C), _ptr(A.rowptr), _ptr(A.colind),
+         A.nblockrows, A.nblocks, int(accumulate), _stream())
+    return C
+
+
+def bmv_bin_bin_bin_masked(A, xb, mask, out=None):
+    C = _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, False)
+    call("bgb_bmv_bin_bin_bin_masked", A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr),
+         _ptr(A.colind), A.nblockrows, A.nblocks, _ptr(mask), _stream())
+    return C
+
+
+def padded_vector(x, T, device="cuda"):
+    """x (host, float32) zero-padded to NBR*T on the device (the drivers' fB)."""
+    n = x.shape[0]
+    nbr = (n + T - 1) // T
+    out = torch.zeros(nbr * T, dtype=torch.float32, device=device)
+    out[:n] = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
+    return out

@@ -1,0 +1,80 @@
+"""Host-side CSR graphs from libbgb's generators (numpy views of int32 CSR)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import call, lib
+
+
+class HostCSR:
+    """An int32 CSR matrix on the host (sorted, duplicate-free rows)."""
+
+    def __init__(self, nrows, ncols, rowptr, colind):
+        self.nrows = int(nrows)
+        self.ncols = int(ncols)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.int32)
+        assert self.rowptr.shape == (self.nrows + 1,)
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    @classmethod
+    def _from_handle(cls, h):
+        L = lib()
+        try:
+            n, m, nnz = L.bgb_csr_nrows(h), L.bgb_csr_ncols(h), L.bgb_csr_nnz(h)
+            rp = np.ctypeslib.as_array(ctypes.cast(L.bgb_csr_rowptr(h), ctypes.POINTER(ctypes.c_int32)),
+                                       shape=(n + 1,)).copy()
+            if nnz > 0:
+                ci = np.ctypeslib.as_array(ctypes.cast(L.bgb_csr_colind(h), ctypes.POINTER(ctypes.c_int32)),
+                                           shape=(nnz,)).copy()
+            else:
+                ci = np.zeros(0, np.int32)
+        finally:
+            L.bgb_csr_free(h)
+        return cls(n, m, rp, ci)
+
+    def degrees(self):
+        return np.diff(self.rowptr)
+
+
+def _gen(fn, *args):
+    h = ctypes.c_void_p()
+    call(fn, *args, ctypes.byref(h))
+    return HostCSR._from_handle(h)
+
+
+def from_coo(nrows, ncols, src, dst, undirected=True, drop_self_loops=True):
+    """Normalised CSR from an edge list (the reference's readMtx.hpp:152-306 rules)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    assert src.shape == dst.shape
+    return _gen("bgb_csr_from_coo", nrows, ncols, src.size, src.ctypes.data, dst.ctypes.data,
+                int(undirected), int(drop_self_loops))
+
+
+def erdos_renyi(n, avg_degree, seed):
+    """G(n, m) with m = n*avg_degree/2 distinct undirected edges, no self-loops."""
+    return _gen("bgb_gen_er", n, int(n * avg_degree) // 2, seed)
+
+
+def rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1):
+    return _gen("bgb_gen_rmat", scale, edge_factor, a, b, c, seed)
+
+
+def grid2d(width, height):
+    return _gen("bgb_gen_grid2d", width, height)
+
+
+def bernoulli(n, p, seed):
+    out = np.empty(n, np.float32)
+    call("bgb_gen_bernoulli_f32", n, p, seed, out.ctypes.data)
+    return out
+
+
+def uniform(n, lo, hi, seed):
+    out = np.empty(n, np.float32)
+    call("bgb_gen_uniform_f32", n, lo, hi, seed, out.ctypes.data)
+    return out

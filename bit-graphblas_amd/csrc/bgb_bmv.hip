@@ -1,0 +1,406 @@
+// B2SR binary SpMV (BMV) kernels for gfx950: bin_bin_bin, bin_bin_full (int32
+// and float counts), bin_full_full (FP32) and masked bin_bin_bin.
+//
+// Semantics follow bsrbmv.cu:274-1229: for block row I, row r of the result
+// combines row r of every tile (I, J) with the x segment of block column J
+// (a packed word for the bin_bin ops, T floats for bin_full_full).
+//
+// Work decomposition (not the reference's one-warp-per-block-row loop,
+// bsrbmv.cu:1046, which serialises R-MAT hub rows):
+//   * the tile array is cut into fixed chunks of CH = (64/L)*S tiles, one
+//     chunk per wave, so every wave streams the same number of bytes;
+//   * inside a chunk each lane group walks S consecutive tiles, accumulating
+//     per-row partials and flushing a row to C whenever the row ends (rows
+//     that end at the same position -- empty rows -- are flushed as zeros);
+//   * the first row a group flushes may have started in an earlier group: a
+//     wave-level segmented scan over the groups' open rows supplies that
+//     carry, so rows that start and end inside one wave are written once;
+//   * a row still open at the end of the chunk leaves a carry record; a
+//     fix-up pass adds those records to C (integer atomics are exact, bit
+//     results use atomic OR, FP32 uses float atomics as the reference does).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+#include <map>
+#include <mutex>
+
+#include "bgb_device.h"
+#include "bgb_tile.h"
+
+namespace bgb {
+
+__device__ __forceinline__ long long lmin(long long a, long long b) { return a < b ? a : b; }
+
+enum BmvOp : int { OP_BITS = 0, OP_MASKED = 1, OP_COUNT_I32 = 2, OP_COUNT_F32 = 3, OP_FULL_F32 = 4 };
+
+struct BmvArgs {
+  const void *A;
+  const void *B;
+  void *C;
+  const int *rowptr;
+  const int *colind;
+  const void *mask;
+  int nbr;
+  int nblocks;
+  int accumulate;
+  int nchunks;
+  int *carry_key;
+  void *carry_val;
+};
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kRun = 8;  // tiles per lane-group run (S)
+
+template <int OP>
+struct AccOf {
+  using type = uint32_t;
+};
+template <>
+struct AccOf<OP_FULL_F32> {
+  using type = float;
+};
+
+template <int T, int OP>
+__device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
+                                          const typename AccOf<OP>::type (&v)[Tile<T>::R]) {
+  constexpr int R = Tile<T>::R, L = Tile<T>::L;
+  using W = typename Tile<T>::W;
+  if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) bits |= (v[i] != 0u ? 1u : 0u) << (T - 1 - (q * R + i));
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
+    if (q == 0) {
+      W *C = static_cast<W *>(a.C);
+      if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[I];
+      else if (a.accumulate) bits |= (uint32_t)C[I];
+      C[I] = (W)bits;
+    }
+  } else {
+    using V = typename std::conditional<OP == OP_COUNT_I32, int, float>::type;
+    V out[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) out[i] = (V)v[i];
+    V *C = static_cast<V *>(a.C) + (long long)I * T + q * R;
+    static_assert(R % 4 == 0, "rows per lane must fill 16-byte stores");
+#pragma unroll
+    for (int j = 0; j < R / 4; ++j) {
+      typedef V V4 __attribute__((ext_vector_type(4)));
+      V4 *p = reinterpret_cast<V4 *>(C) + j;
+      V4 w = {out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]};
+      if (a.accumulate) w += *p;
+      *p = w;
+    }
+  }
+}
+
+template <int T, int OP, int S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
+  constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
+  using Acc = typename AccOf<OP>::type;
+  using W = typename Tile<T>::W;
+  const int lane = threadIdx.x & 63;
+  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (chunk >= a.nchunks) return;  // uniform per wave
+  const int g = lane / L, q = lane % L;
+  const long long k0 = (long long)chunk * CH;
+  const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
+  const long long pb = lmin(pa + S, a.nblocks);
+  const bool active = pa < pb;
+  const int *__restrict__ rowptr = a.rowptr;
+
+  // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
+  int I = 0;
+  if (active) {
+    int lo = 0, hi = a.nbr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rowptr[mid] <= pa) lo = mid;
+      else hi = mid - 1;
+    }
+    I = lo;
+  }
+  const int I0 = I;
+
+  uint32_t F[S][NW];
+  int J[S];
+  const int n = (int)(pb - pa);
+  if (active) load_run<T, S>(a.A, a.colind, pa, n, q, F, J);
+
+  uint32_t X[S];
+  if constexpr (OP != OP_FULL_F32) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
+  }
+
+  Acc acc[R], zero[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) acc[i] = zero[i] = (Acc)0;
+
+  // Leading empty block rows end at position 0: the group holding tile 0 owns them.
+  if (active && pa == 0)
+    for (int r = 0; r < I0; ++r) flush_row<T, OP>(a, r, q, zero);
+
+  int held = -1;
+  Acc hv[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) hv[i] = (Acc)0;
+  int e = active ? rowptr[I + 1] : INT_MAX;
+
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (s < n) {
+      if constexpr (OP == OP_FULL_F32) {
+        const float *__restrict__ x = static_cast<const float *>(a.B) + (long long)J[s] * T;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          uint32_t w = row_word<T>(F[s], i);
+          while (w) {
+            const int h = 31 - __clz(w);
+            acc[i] += x[T - 1 - h];
+            w &= ~(1u << h);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i] += __popc(row_word<T>(F[s], i) & X[s]);
+      }
+      const int pos = (int)(pa + s) + 1;
+      if (pos == e) {
+        if (held < 0) {
+          held = I;
+#pragma unroll
+          for (int i = 0; i < R; ++i) hv[i] = acc[i];
+        } else {
+          flush_row<T, OP>(a, I, q, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
+        ++I;
+        e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
+        while (e == pos) {
+          flush_row<T, OP>(a, I, q, zero);
+          ++I;
+          e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
+        }
+      }
+    }
+  }
+
+  // Segmented inclusive scan of the open-row partials across groups (keys
+  // are non-decreasing over groups, so equal keys are contiguous).
+  const int key = active ? I : INT_MAX;
+  Acc sc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) sc[i] = acc[i];
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int ku = __shfl_up(key, d * L);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const Acc up = __shfl_up(sc[i], d * L);
+      if (g >= d && ku == key) sc[i] += up;
+    }
+  }
+  // Carry into this group's first flushed row from the groups before it.
+  const int kin = __shfl_up(key, L);
+  Acc cin[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const Acc up = __shfl_up(sc[i], L);
+    cin[i] = (g > 0 && kin == I0) ? up : (Acc)0;
+  }
+  if (held >= 0) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) hv[i] += cin[i];
+    flush_row<T, OP>(a, held, q, hv);
+  }
+  // The last active group publishes the chunk's open row.
+  const int last = (int)lmin(G - 1, (a.nblocks - 1 - k0) / S);
+  if (g == last) {
+    const bool open = key < a.nbr;
+    if (q == 0) a.carry_key[chunk] = open ? key : -1;
+    if (open) {
+      Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
+#pragma unroll
+      for (int i = 0; i < R; ++i) cv[i] = sc[i];
+    }
+  }
+}
+
+// Adds the chunk carries of rows that cross chunk boundaries.
+template <int T, int OP>
+__global__ void bmv_fixup_kernel(BmvArgs a) {
+  using Acc = typename AccOf<OP>::type;
+  using W = typename Tile<T>::W;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+    if (tid >= a.nchunks) return;
+    const int key = a.carry_key[tid];
+    if (key < 0) return;
+    const Acc *cv = static_cast<const Acc *>(a.carry_val) + tid * T;
+    uint32_t bits = 0;
+    for (int r = 0; r < T; ++r) bits |= (cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
+    if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[key];
+    if (bits == 0) return;
+    // OR into the aligned dword holding word `key` (W is 1, 2 or 4 bytes).
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(a.C) + key);
+    const int shift = (int)(addr & 3) * 8;
+    atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << shift);
+  } else {
+    if (tid >= (long long)a.nchunks * T) return;
+    const long long c = tid / T;
+    const int r = (int)(tid % T);
+    const int key = a.carry_key[c];
+    if (key < 0) return;
+    const Acc v = static_cast<const Acc *>(a.carry_val)[tid];
+    if (v == (Acc)0) return;
+    if constexpr (OP == OP_COUNT_I32) atomicAdd(static_cast<int *>(a.C) + (long long)key * T + r, (int)v);
+    else unsafeAtomicAdd(static_cast<float *>(a.C) + (long long)key * T + r, (float)v);
+  }
+}
+
+template <class V>
+__global__ void fill_kernel(V *p, long long n, V v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <int T, int OP>
+int launch_bmv(BmvArgs a, hipStream_t s) {
+  constexpr int S = kRun, CH = (64 / Tile<T>::L) * S;
+  using Acc = typename AccOf<OP>::type;
+  const long long nchunks = ceil_div(a.nblocks, CH);
+  BGB_REQUIRE(nchunks < INT_MAX / 2, BGB_ERR_SIZE, "too many chunks");
+  a.nchunks = (int)nchunks;
+  void *ws = nullptr;
+  const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
+  int st = stream_workspace(s, key_bytes + sizeof(Acc) * (size_t)nchunks * T, &ws);
+  if (st) return st;
+  a.carry_key = static_cast<int *>(ws);
+  a.carry_val = static_cast<char *>(ws) + key_bytes;
+  const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
+  bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
+  BGB_HIP_TRY(hipGetLastError());
+  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : nchunks * T;
+  bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
+  BGB_HIP_TRY(hipGetLastError());
+  return BGB_OK;
+}
+
+template <int OP>
+int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, const int *colind,
+              int nbr, int nblocks, int accumulate, const void *mask, bgb_stream_t stream) {
+  clear_error();
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(nbr >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
+  BGB_REQUIRE(C && rowptr, BGB_ERR_ARG, "null output or rowptr");
+  BGB_REQUIRE(nblocks == 0 || (A && B && colind), BGB_ERR_ARG, "null matrix / vector");
+  BGB_REQUIRE(OP != OP_MASKED || mask, BGB_ERR_ARG, "null mask");
+  BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
+  BGB_REQUIRE((long long)nbr * T < INT_MAX, BGB_ERR_SIZE, "NBR*T exceeds the int32 range");
+  auto aligned16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  BGB_REQUIRE(aligned16(A) && aligned16(colind), BGB_ERR_ARG,
+              "bsrVal and bsrColInd must be 16-byte aligned");
+  BGB_REQUIRE(OP == OP_BITS || OP == OP_MASKED || aligned16(C), BGB_ERR_ARG,
+              "count / float outputs must be 16-byte aligned");
+  if (nbr == 0) return BGB_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nblocks == 0) {  // every block row is empty
+    if (accumulate && OP != OP_MASKED) return BGB_OK;
+    const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)nbr * word_bytes(T)
+                                                           : (long long)nbr * T * 4;
+    fill_kernel<uint8_t><<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(static_cast<uint8_t *>(C), n, 0);
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  }
+  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr};
+  switch (T) {
+    case 4: return launch_bmv<4, OP>(a, s);
+    case 8: return launch_bmv<8, OP>(a, s);
+    case 16: return launch_bmv<16, OP>(a, s);
+    default: return launch_bmv<32, OP>(a, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-(device, stream) scratch.
+namespace {
+struct Ws {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, Ws> g_ws;
+}  // namespace
+
+int stream_workspace(hipStream_t s, size_t bytes, void **out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Ws &w = g_ws[{dev, s}];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      // the previous buffer may still be read by queued work on this stream
+      BGB_HIP_TRY(hipStreamSynchronize(s));
+      BGB_HIP_TRY(hipFree(w.ptr));
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    const size_t want = std::max(bytes + bytes / 4, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&w.ptr, want);
+    if (e != hipSuccess) {
+      w.ptr = nullptr;
+      return set_error(BGB_ERR_ALLOC, "workspace hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    w.bytes = want;
+  }
+  *out = w.ptr;
+  return BGB_OK;
+}
+
+}  // namespace bgb
+
+using namespace bgb;
+
+extern "C" {
+
+int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C, const int *d_rowptr,
+                        const int *d_colind, int nblockrows, int nblocks, int accumulate,
+                        bgb_stream_t stream) {
+  return bmv_entry<OP_BITS>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks, accumulate,
+                            nullptr, stream);
+}
+
+int bgb_bmv_bin_bin_full_i32(int T, const void *d_A, const void *d_B, int *d_C, const int *d_rowptr,
+                             const int *d_colind, int nblockrows, int nblocks, int accumulate,
+                             bgb_stream_t stream) {
+  return bmv_entry<OP_COUNT_I32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
+                                 accumulate, nullptr, stream);
+}
+
+int bgb_bmv_bin_bin_full_f32(int T, const void *d_A, const void *d_B, float *d_C,
+                             const int *d_rowptr, const int *d_colind, int nblockrows, int nblocks,
+                             int accumulate, bgb_stream_t stream) {
+  return bmv_entry<OP_COUNT_F32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
+                                 accumulate, nullptr, stream);
+}
+
+int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C, const int *d_rowptr,
+                          const int *d_colind, int nblockrows, int nblocks, int accumulate,
+                          bgb_stream_t stream) {
+  return bmv_entry<OP_FULL_F32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
+                                accumulate, nullptr, stream);
+}
+
+int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_C,
+                               const int *d_rowptr, const int *d_colind, int nblockrows,
+                               int nblocks, const void *d_mask, bgb_stream_t stream) {
+  return bmv_entry<OP_MASKED>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks, 0, d_mask,
+                              stream);
+}
+
+}  // extern "C"

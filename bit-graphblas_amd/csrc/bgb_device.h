@@ -1,0 +1,45 @@
+// Device-side helpers shared by the HIP translation units of libbgb.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bgb_internal.h"
+
+#define BGB_HIP_TRY(expr)                                                         \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess)                                                         \
+      return ::bgb::set_error(BGB_ERR_HIP, "%s failed: %s (%s:%d)", #expr,        \
+                              hipGetErrorString(_e), __FILE__, __LINE__);         \
+  } while (0)
+
+namespace bgb {
+
+// Stream-ordered temporary device allocation, released on the same stream.
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  hipStream_t stream = nullptr;
+  hipError_t alloc(size_t bytes, hipStream_t s) {
+    stream = s;
+    if (bytes == 0) bytes = 256;
+    return hipMallocAsync(&ptr, bytes, s);
+  }
+  ~DeviceBuffer() {
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+  }
+  template <class P>
+  P *as() const { return static_cast<P *>(ptr); }
+};
+
+hipError_t radix_sort_u64(const uint64_t *in, uint64_t *out, size_t n, int end_bit, hipStream_t s);
+hipError_t exclusive_scan_u8_to_i64(const uint8_t *in, long long *out, size_t n, hipStream_t s);
+
+// Per-(device, stream) grow-only scratch used by the BMV carry records. The
+// pointer stays valid until a later call on the same stream asks for more.
+int stream_workspace(hipStream_t s, size_t bytes, void **out);
+
+inline int ilog2_tiledim(int T) { return T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : 5; }
+
+}  // namespace bgb

@@ -1,0 +1,135 @@
+// B2SR tile traits and per-lane fragment access for gfx950 (wave64).
+//
+// A lane group of L lanes owns one tile at a time; each lane holds R of its T
+// rows in NW 32-bit registers (its "fragment"):
+//   T=4 : 1 lane,  4 rows, 1 dword  -- byte r = row r, low nibble, col c at bit 3-c
+//   T=8 : 1 lane,  8 rows, 2 dwords -- byte r = row r, col c at bit 7-c
+//   T=16: 2 lanes, 8 rows, 4 dwords -- half-word i = row 8q+i, col c at bit 15-c
+//   T=32: 8 lanes, 4 rows, 4 dwords -- dword i = row 4q+i, col c at bit 31-c
+// Every lane therefore loads 4, 8 or 16 contiguous bytes per tile and a wave
+// streams 64/L tiles per instruction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bgb {
+
+template <int T>
+struct Tile;
+template <>
+struct Tile<4> {
+  using W = uint8_t;
+  static constexpr int L = 1, R = 4, NW = 1, LG = 2;
+};
+template <>
+struct Tile<8> {
+  using W = uint8_t;
+  static constexpr int L = 1, R = 8, NW = 2, LG = 3;
+};
+template <>
+struct Tile<16> {
+  using W = uint16_t;
+  static constexpr int L = 2, R = 8, NW = 4, LG = 4;
+};
+template <>
+struct Tile<32> {
+  using W = uint32_t;
+  static constexpr int L = 8, R = 4, NW = 4, LG = 5;
+};
+
+// Row i (0 <= i < R) of a lane fragment, as a T-bit word.
+template <int T>
+__device__ __forceinline__ uint32_t row_word(const uint32_t (&F)[Tile<T>::NW], int i) {
+  if constexpr (T == 4) return (F[0] >> (8 * i)) & 0xFu;
+  else if constexpr (T == 8) return (F[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+  else if constexpr (T == 16) return (F[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+  else return F[i];
+}
+
+// Fragment of tile p for lane q of its group.
+template <int T>
+__device__ __forceinline__ void load_frag(const void *__restrict__ A, long long p, int q,
+                                          uint32_t (&F)[Tile<T>::NW]) {
+  if constexpr (T == 4) {
+    F[0] = static_cast<const uint32_t *>(A)[p];
+  } else if constexpr (T == 8) {
+    const uint2 v = static_cast<const uint2 *>(A)[p];
+    F[0] = v.x;
+    F[1] = v.y;
+  } else {
+    constexpr int L = Tile<T>::L;
+    const uint4 v = static_cast<const uint4 *>(A)[p * L + q];
+    F[0] = v.x;
+    F[1] = v.y;
+    F[2] = v.z;
+    F[3] = v.w;
+  }
+}
+
+// Loads the fragments and block columns of tiles [pa, pa+n), n <= S. With
+// n == S and pa a multiple of S (S a multiple of 4) the L=1 tile dims use
+// 16-byte loads over the lane's contiguous run.
+template <int T, int S>
+__device__ __forceinline__ void load_run(const void *__restrict__ A, const int *__restrict__ colind,
+                                         long long pa, int n, int q,
+                                         uint32_t (&F)[S][Tile<T>::NW], int (&J)[S]) {
+  static_assert(S % 4 == 0, "run length must be a multiple of 4");
+  if (n == S) {
+    const int4 *ci = reinterpret_cast<const int4 *>(colind + pa);
+#pragma unroll
+    for (int s = 0; s < S / 4; ++s) {
+      const int4 c = ci[s];
+      J[4 * s] = c.x;
+      J[4 * s + 1] = c.y;
+      J[4 * s + 2] = c.z;
+      J[4 * s + 3] = c.w;
+    }
+    if constexpr (T == 4) {
+      const uint4 *a = static_cast<const uint4 *>(A) + pa / 4;
+#pragma unroll
+      for (int s = 0; s < S / 4; ++s) {
+        const uint4 v = a[s];
+        F[4 * s][0] = v.x;
+        F[4 * s + 1][0] = v.y;
+        F[4 * s + 2][0] = v.z;
+        F[4 * s + 3][0] = v.w;
+      }
+    } else if constexpr (T == 8) {
+      const uint4 *a = static_cast<const uint4 *>(A) + pa / 2;
+#pragma unroll
+      for (int s = 0; s < S / 2; ++s) {
+        const uint4 v = a[s];
+        F[2 * s][0] = v.x;
+        F[2 * s][1] = v.y;
+        F[2 * s + 1][0] = v.z;
+        F[2 * s + 1][1] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) load_frag<T>(A, pa + s, q, F[s]);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (s < n) {
+        J[s] = colind[pa + s];
+        load_frag<T>(A, pa + s, q, F[s]);
+      } else {
+        J[s] = 0;
+#pragma unroll
+        for (int w = 0; w < Tile<T>::NW; ++w) F[s][w] = 0;
+      }
+    }
+  }
+}
+
+// Workgroup index remapped so that consecutive index ranges stay on one XCD
+// (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+}  // namespace bgb

@@ -1,0 +1,164 @@
+/*
+ * bgb.h — C ABI of the MI355X-native B2SR bit-packed binary SpMV (BMV) library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (BGB_OK == 0) and, when it touches the GPU, runs asynchronously on
+ * the caller's HIP stream (`bgb_stream_t`, a hipStream_t; NULL = default stream).
+ * Device pointers are prefixed `d_`, host pointers `h_`.
+ *
+ * B2SR layout (identical to the reference's, bsrbmv.cu:28-272 and
+ * test_bin_bin_full.cu:49-59): tile dim T in {4, 8, 16, 32},
+ *   NBR          = ceil(nrows / T) block rows
+ *   bsrRowPtr    int32[NBR + 1]
+ *   bsrColInd    int32[nblocks]            ascending block column J per block row
+ *   bsrVal       W_T[nblocks * T]          word k*T + r holds row r of tile k,
+ *                                          column c at bit (T-1-c)
+ *   W_T = uint8 (T=4, 8), uint16 (T=16), uint32 (T=32); for T=4 the high nibble is 0.
+ * Packed bit vector (ToBit{T}Row, bsrbmv.cu:65-222): W_T[ceil(NBR/T)*T] words,
+ *   word J holds x[J*T + c] > 0 at bit (T-1-c); for T=4 one nibble per byte.
+ * Bit output of bin_bin_bin: W_T[NBR], word I holds row I*T + r at bit (T-1-r).
+ * Count / FP32 outputs: [NBR * T] values, element I*T + r.
+ */
+#ifndef BGB_H
+#define BGB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *bgb_stream_t; /* hipStream_t */
+
+enum {
+  BGB_OK = 0,
+  BGB_ERR_ARG = 1,      /* null pointer, negative size, bad flag */
+  BGB_ERR_TILEDIM = 2,  /* T not in {4, 8, 16, 32} */
+  BGB_ERR_HIP = 3,      /* a HIP runtime call failed */
+  BGB_ERR_ALLOC = 4,    /* device / host allocation failed */
+  BGB_ERR_SIZE = 5,     /* a size exceeds the int32 index range or a stated limit */
+  BGB_ERR_FORMAT = 6    /* input structure invalid (non-monotone rowptr, ...) */
+};
+
+/* Library information / errors. bgb_last_error() is per host thread. */
+int bgb_version(void);
+const char *bgb_status_string(int status);
+const char *bgb_last_error(void);
+/* Number of W_T words of a packed vector for NBR block rows: ceil(NBR/T)*T
+ * (the reference's allocation, test_bin_bin_full.cu:246-266). */
+long long bgb_packed_vec_words(int T, int nblockrows);
+/* sizeof(W_T) in bytes; 0 if T is invalid. */
+int bgb_word_bytes(int T);
+
+/* ------------------------------------------------------------------------ *
+ * CSR -> B2SR packing (GPU).                                                 *
+ * ------------------------------------------------------------------------ */
+
+/* Block-row tile counts: fills d_bsrRowPtr[NBR+1] and *h_nblocks.
+ * Replaces cusparseXcsr2bsrNnz (test_bin_bin_full.cu:199-200).
+ * Input CSR must have ascending, duplicate-free column indices per row.
+ * Synchronises `stream` (nblocks is returned to the host). */
+int bgb_b2sr_nnzb(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
+                  const int *d_csrColInd, int *d_bsrRowPtr, int *h_nblocks,
+                  bgb_stream_t stream);
+
+/* Fills d_bsrColInd[nblocks] and d_bsrVal[nblocks*T] given the row pointer
+ * from bgb_b2sr_nnzb. Replaces csr2bsr_batch_{4,8,16,32}
+ * (csr2bsr_batch_bsrbmv.cu:16,140,263,386) and the cusparseScsr2bsr +
+ * ToBit{T}Col pipeline it drives (csr2bsr_batch_bsrbmv.cu:100-107). */
+int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
+                 const int *d_csrColInd, const int *d_bsrRowPtr, int nblocks,
+                 int *d_bsrColInd, void *d_bsrVal, bgb_stream_t stream);
+
+/* Packs a float vector into B2SR bit words: word J bit (T-1-c) = x[J*T+c] > 0.
+ * Reads x[0..n) (entries at or beyond n count as 0) and writes
+ * bgb_packed_vec_words(T, ceil(n/T)) words. Replaces ToBit{T}Row
+ * (bsrbmv.cu:65,117,163,206). */
+int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * BMV (GPU). A = bsrVal, rowptr/colind = bsrRowPtr/bsrColInd, nblocks =     *
+ * rowptr[nblockrows] (passed so that no device read is needed).             *
+ * accumulate = 1 reproduces the reference's in-place accumulation           *
+ * (|= for bits, += for counts and sums); 0 overwrites C.                    *
+ * Calls on one stream are ordered; calls that overlap on different streams  *
+ * use separate per-stream workspaces.                                       *
+ * ------------------------------------------------------------------------ */
+
+/* C[I] (|)= bits(T-1-r) = OR_c A[I*T+r][c] & x[c].  bmv{T}_bin_bin_bin
+ * (bsrbmv.cu:277,437,508,579). B: packed bit vector, C: W_T[NBR]. */
+int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C,
+                        const int *d_rowptr, const int *d_colind, int nblockrows,
+                        int nblocks, int accumulate, bgb_stream_t stream);
+
+/* C[I*T+r] (+)= popcount(A row & x), exact int32.  bmv{T}_bin_bin_full
+ * (bsrbmv.cu:618,678,738,789) with an int32 output. */
+int bgb_bmv_bin_bin_full_i32(int T, const void *d_A, const void *d_B, int *d_C,
+                             const int *d_rowptr, const int *d_colind,
+                             int nblockrows, int nblocks, int accumulate,
+                             bgb_stream_t stream);
+
+/* Same counts stored as float, the reference's output type
+ * (test_bin_bin_full.cu:62,314,335-346). Exact below 2^24. */
+int bgb_bmv_bin_bin_full_f32(int T, const void *d_A, const void *d_B, float *d_C,
+                             const int *d_rowptr, const int *d_colind,
+                             int nblockrows, int nblocks, int accumulate,
+                             bgb_stream_t stream);
+
+/* C[I*T+r] (+)= sum over tiles (I,J), columns c of A[I*T+r][J*T+c] * x[J*T+c],
+ * FP32; B: float[NBR*T]
+ * (padded with zeros beyond nrows).  bmv{T}_bin_full_full
+ * (bsrbmv.cu:826,899,975,1040). Each row is summed in ascending column order
+ * inside a 64-lane chunk; rows that cross chunks are combined with FP32
+ * atomics (the reference also uses atomicAdd, bsrbmv.cu:895,971,1036). */
+int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C,
+                          const int *d_rowptr, const int *d_colind, int nblockrows,
+                          int nblocks, int accumulate, bgb_stream_t stream);
+
+/* C[I] = bits & ~mask[I] (always overwrites).  bmv{T}_bin_bin_bin_masked
+ * (bsrbmv.cu:1086,1122,1158,1194), with the reference's T=8/16 shift and
+ * nibble-mask defects corrected (the mask applies to all T bits). */
+int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_C,
+                               const int *d_rowptr, const int *d_colind,
+                               int nblockrows, int nblocks, const void *d_mask,
+                               bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * Host graph generators and CSR normalisation (no reference counterpart    *
+ * except the normalisation, readMtx.hpp:152-306: symmetrise, drop           *
+ * self-loops and duplicates, sort by (row, col), COO -> CSR).                *
+ * All random streams are counter-based splitmix64, so results do not depend *
+ * on the thread count.                                                      *
+ * ------------------------------------------------------------------------ */
+typedef struct bgb_csr bgb_csr; /* opaque host CSR (int32 indices) */
+
+int bgb_csr_from_coo(int nrows, int ncols, long long nedges, const int *h_src,
+                     const int *h_dst, int undirected, int drop_self_loops,
+                     bgb_csr **out);
+/* Erdos-Renyi G(n, m): exactly m distinct undirected edges, no self-loops. */
+int bgb_gen_er(int n, long long m, uint64_t seed, bgb_csr **out);
+/* R-MAT(scale, edge_factor*2^scale edges, a, b, c, d = 1-a-b-c), no noise or
+ * permutation, symmetrised, self-loops and duplicates dropped. */
+int bgb_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                 uint64_t seed, bgb_csr **out);
+/* width x height 4-neighbour grid, vertex id y*width + x. */
+int bgb_gen_grid2d(int width, int height, bgb_csr **out);
+int bgb_csr_nrows(const bgb_csr *g);
+int bgb_csr_ncols(const bgb_csr *g);
+long long bgb_csr_nnz(const bgb_csr *g);
+const int *bgb_csr_rowptr(const bgb_csr *g);
+const int *bgb_csr_colind(const bgb_csr *g);
+void bgb_csr_free(bgb_csr *g);
+
+/* Deterministic vectors: out[i] = (u_i < p) ? 1 : 0, or lo + (hi-lo)*u_i,
+ * u_i = uniform [0,1) from (seed, i). */
+int bgb_gen_bernoulli_f32(long long n, double p, uint64_t seed, float *h_out);
+int bgb_gen_uniform_f32(long long n, double lo, double hi, uint64_t seed,
+                        float *h_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BGB_H */

@@ -1,0 +1,192 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Built by ``make -C oracle`` (``__graft_entry__.build()`` does it). The
+reference-built cross-check oracle/_ref/libreadmtx_ref.so is optional.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+REF_SO = os.path.join(REPO, "oracle", "_ref", "libreadmtx_ref.so")
+
+WORD_NP = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32}
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        self.L = ctypes.CDLL(path)
+        vp, ci, ll, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+        sig = {
+            "orc_csr_normalize": (ci, [ci, ci, ll, vp, vp, ci, ci, vp, vp, vp]),
+            "orc_b2sr_nnzb": (ci, [ci, ci, ci, vp, vp, vp]),
+            "orc_b2sr_fill": (None, [ci, ci, ci, vp, vp, vp, vp, vp]),
+            "orc_pack_vec": (None, [ci, vp, ci, ll, vp]),
+            "orc_pack_u8": (None, [ci, vp, ci, ll, vp]),
+            "orc_b2sr_bmv_bin_bin_bin": (None, [ci, vp, vp, vp, vp, vp, ci]),
+            "orc_b2sr_bmv_bin_bin_full": (None, [ci, vp, vp, vp, vp, vp, ci]),
+            "orc_b2sr_bmv_bin_full_full": (None, [ci, vp, vp, vp, vp, vp, ci]),
+            "orc_b2sr_bmv_masked": (None, [ci, vp, vp, vp, vp, vp, ci, vp]),
+            "orc_csr_spmv_count": (None, [ci, vp, vp, vp, vp]),
+            "orc_csr_spmv_or": (None, [ci, vp, vp, vp, vp]),
+            "orc_csr_spmv_f32": (None, [ci, vp, vp, vp, vp]),
+            "orc_csr_spmv_f64": (None, [ci, vp, vp, vp, vp]),
+            "orc_csr_spmm_f64": (None, [ci, vp, vp, ci, vp, vp]),
+            "orc_bfs": (ci, [ci, vp, vp, ci, vp]),
+            "orc_pagerank_f64": (None, [ci, vp, vp, dbl, ci, vp, vp]),
+            "orc_num_threads": (ci, []),
+            "orc_set_threads": (None, [ci]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(self.L, name)
+            f.restype, f.argtypes = res, args
+
+    # -- normalisation / packing -------------------------------------------
+    def normalize(self, nrows, ncols, src, dst, undirected=True, drop_self=True):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        rp = np.zeros(nrows + 1, np.int32)
+        ci = np.zeros(max(2 * src.size, 1), np.int32)
+        nnz = ctypes.c_longlong()
+        st = self.L.orc_csr_normalize(nrows, ncols, src.size, _p(src), _p(dst), int(undirected),
+                                      int(drop_self), _p(rp), _p(ci), ctypes.byref(nnz))
+        assert st == 0, "edge index out of range"
+        return rp, ci[: nnz.value].copy()
+
+    def b2sr(self, T, nrows, ncols, rowptr, colind):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        colind = np.ascontiguousarray(colind if len(colind) else np.zeros(1), np.int32)
+        nbr = (nrows + T - 1) // T
+        brp = np.zeros(nbr + 1, np.int32)
+        nb = self.L.orc_b2sr_nnzb(T, nrows, ncols, _p(rowptr), _p(colind), _p(brp))
+        bci = np.zeros(max(nb, 1), np.int32)
+        val = np.zeros(max(nb, 1) * T, WORD_NP[T])
+        self.L.orc_b2sr_fill(T, nrows, ncols, _p(rowptr), _p(colind), _p(brp), _p(bci), _p(val))
+        return brp, bci[:nb], val[: nb * T]
+
+    @staticmethod
+    def nwords(T, nbr):
+        return -(-nbr // T) * T
+
+    def pack_vec(self, T, x, nbr):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(max(self.nwords(T, nbr), 1), WORD_NP[T])
+        self.L.orc_pack_vec(T, _p(x), x.size, self.nwords(T, nbr), _p(out))
+        return out[: self.nwords(T, nbr)]
+
+    def pack_u8(self, T, bits, nwords):
+        bits = np.ascontiguousarray(bits, np.uint8)
+        out = np.zeros(max(nwords, 1), WORD_NP[T])
+        self.L.orc_pack_u8(T, _p(bits), bits.size, nwords, _p(out))
+        return out[:nwords]
+
+    # -- B2SR tile loops -----------------------------------------------------
+    def b2sr_bin_bin_bin(self, T, val, xb, rowptr, colind, C=None):
+        nbr = rowptr.size - 1
+        C = np.zeros(nbr, WORD_NP[T]) if C is None else C.copy()
+        self.L.orc_b2sr_bmv_bin_bin_bin(T, _p(val), _p(xb), _p(C), _p(rowptr), _p(colind), nbr)
+        return C
+
+    def b2sr_bin_bin_full(self, T, val, xb, rowptr, colind, C=None):
+        nbr = rowptr.size - 1
+        C = np.zeros(nbr * T, np.int32) if C is None else C.astype(np.int32).copy()
+        self.L.orc_b2sr_bmv_bin_bin_full(T, _p(val), _p(xb), _p(C), _p(rowptr), _p(colind), nbr)
+        return C
+
+    def b2sr_bin_full_full(self, T, val, x, rowptr, colind, C=None):
+        nbr = rowptr.size - 1
+        x = np.ascontiguousarray(x, np.float32)
+        C = np.zeros(nbr * T, np.float32) if C is None else C.astype(np.float32).copy()
+        self.L.orc_b2sr_bmv_bin_full_full(T, _p(val), _p(x), _p(C), _p(rowptr), _p(colind), nbr)
+        return C
+
+    def b2sr_masked(self, T, val, xb, rowptr, colind, mask):
+        nbr = rowptr.size - 1
+        C = np.zeros(nbr, WORD_NP[T])
+        mask = np.ascontiguousarray(mask, WORD_NP[T])
+        self.L.orc_b2sr_bmv_masked(T, _p(val), _p(xb), _p(C), _p(rowptr), _p(colind), nbr, _p(mask))
+        return C
+
+    # -- CSR semirings -------------------------------------------------------
+    def csr_count(self, rowptr, colind, x01):
+        n = rowptr.size - 1
+        x01 = np.ascontiguousarray(x01, np.uint8)
+        y = np.zeros(n, np.int32)
+        self.L.orc_csr_spmv_count(n, _p(rowptr), _p(colind), _p(x01), _p(y))
+        return y
+
+    def csr_or(self, rowptr, colind, x01):
+        n = rowptr.size - 1
+        x01 = np.ascontiguousarray(x01, np.uint8)
+        y = np.zeros(n, np.uint8)
+        self.L.orc_csr_spmv_or(n, _p(rowptr), _p(colind), _p(x01), _p(y))
+        return y
+
+    def csr_f32(self, rowptr, colind, x):
+        n = rowptr.size - 1
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(n, np.float32)
+        self.L.orc_csr_spmv_f32(n, _p(rowptr), _p(colind), _p(x), _p(y))
+        return y
+
+    def csr_f64(self, rowptr, colind, x):
+        n = rowptr.size - 1
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(n, np.float64)
+        self.L.orc_csr_spmv_f64(n, _p(rowptr), _p(colind), _p(x), _p(y))
+        return y
+
+    def csr_spmm_f64(self, rowptr, colind, X):
+        n = rowptr.size - 1
+        X = np.ascontiguousarray(X, np.float64)
+        Y = np.zeros((n, X.shape[1]), np.float64)
+        self.L.orc_csr_spmm_f64(n, _p(rowptr), _p(colind), X.shape[1], _p(X), _p(Y))
+        return Y
+
+    def bfs(self, rowptr, colind, src):
+        n = rowptr.size - 1
+        lev = np.zeros(n, np.int32)
+        nl = self.L.orc_bfs(n, _p(rowptr), _p(colind), int(src), _p(lev))
+        return lev, nl
+
+    def pagerank(self, rowptr, colind, damping, iters):
+        n = rowptr.size - 1
+        pr = np.zeros(n, np.float64)
+        res = np.zeros(max(iters, 1), np.float64)
+        self.L.orc_pagerank_f64(n, _p(rowptr), _p(colind), damping, iters, _p(pr), _p(res))
+        return pr, res[:iters]
+
+    def threads(self):
+        return self.L.orc_num_threads()
+
+    def set_threads(self, t):
+        self.L.orc_set_threads(int(t))
+
+
+class RefReadMtx:
+    """The reference's own readMtx.hpp CSR construction, built into oracle/_ref."""
+
+    def __init__(self, path=REF_SO):
+        self.L = ctypes.CDLL(path)
+        vp, ci, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+        f = self.L.ref_normalize
+        f.restype, f.argtypes = ci, [ci, ci, ll, vp, vp, ci, vp, vp, vp]
+
+    def normalize(self, nrows, ncols, src, dst, undirected=True):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        rp = np.zeros(nrows + 1, np.int32)
+        ci = np.zeros(max(2 * src.size, 1), np.int32)
+        nnz = ctypes.c_longlong()
+        st = self.L.ref_normalize(nrows, ncols, src.size, _p(src), _p(dst), int(undirected), _p(rp),
+                                  _p(ci), ctypes.byref(nnz))
+        assert st == 0
+        return rp, ci[: nnz.value].copy()

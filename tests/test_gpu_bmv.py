@@ -1,0 +1,212 @@
+"""GPU parity of CSR->B2SR packing and the BMV operators against the CPU oracle.
+
+Bit and count results must match bit-exactly; bin_full_full (FP32) within
+1e-6 relative of the FP64 CSR sum (plus an absolute 1e-6 floor for values
+near zero, the reference's checkResult tolerance, utility.cu:137-149).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TS = (4, 8, 16, 32)
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "b2sr_5x5.json")
+FP_RTOL, FP_ATOL = 1e-6, 1e-6
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev(cuda):
+    import bgb.b2sr  # noqa: F401
+    return cuda
+
+
+def to_np(t, T=None):
+    a = t.cpu().numpy()
+    if T is not None:
+        from bgb.b2sr import WORD_NUMPY
+        a = a.view(WORD_NUMPY[T])
+    return a
+
+
+def check_all_ops(orc, csr, T, rng, dev, label, xdensity=0.3):
+    """Packs csr on the GPU and checks every operator against the oracle."""
+    import torch
+    from bgb import b2sr as B
+    log(f"[case] {label} T={T} n={csr.nrows} nnz={csr.nnz}")
+    n = csr.nrows
+    A = B.B2SR.from_csr(csr, T, dev)
+    brp, bci, val = orc.b2sr(T, n, csr.ncols, csr.rowptr, csr.colind)
+    g_rp, g_ci, g_val = A.to_host()
+    np.testing.assert_array_equal(g_rp, brp)
+    np.testing.assert_array_equal(g_ci, bci)
+    np.testing.assert_array_equal(g_val, val)
+    nbr = A.nblockrows
+    x01 = (rng.random(n) < xdensity).astype(np.float32)
+    xpad = B.padded_vector(x01, T, dev)
+    xb = B.pack_vec(xpad, T)
+    want_xb = orc.pack_vec(T, x01, nbr)
+    np.testing.assert_array_equal(to_np(xb, T)[: want_xb.size], want_xb)
+
+    counts = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+    got = to_np(B.bmv_bin_bin_full(A, xb))
+    np.testing.assert_array_equal(got[:n], counts)
+    assert not got[n:].any()
+    gotf = to_np(B.bmv_bin_bin_full(A, xb, dtype=torch.float32))
+    np.testing.assert_array_equal(gotf[:n], counts.astype(np.float32))
+
+    want_bits = orc.pack_u8(T, (counts > 0).astype(np.uint8), nbr)
+    bits = to_np(B.bmv_bin_bin_bin(A, xb), T)
+    np.testing.assert_array_equal(bits, want_bits)
+
+    mbits = (rng.random(n) < 0.5).astype(np.uint8)
+    mask = torch.from_numpy(orc.pack_u8(T, mbits, nbr).view(to_np(xb).dtype)).to(dev)
+    got_m = to_np(B.bmv_bin_bin_bin_masked(A, xb, mask), T)
+    want_m = orc.pack_u8(T, ((counts > 0) & (mbits == 0)).astype(np.uint8), nbr)
+    np.testing.assert_array_equal(got_m, want_m)
+
+    xf = rng.random(n).astype(np.float32)
+    xfp = B.padded_vector(xf, T, dev)
+    gff = to_np(B.bmv_bin_full_full(A, xfp))
+    ref64 = orc.csr_f64(csr.rowptr, csr.colind, xf.astype(np.float64))
+    np.testing.assert_allclose(gff[:n], ref64, rtol=FP_RTOL, atol=FP_ATOL)
+    assert not gff[n:].any()
+    return A, xb, xfp
+
+
+def test_golden_fixture(orc, dev):
+    import torch
+    from bgb import b2sr as B
+    from bgb.graph import HostCSR
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    n = gold["n"]
+    csr = HostCSR(n, n, gold["csr_rowptr"], gold["csr_colind"])
+    for T in TS:
+        log(f"[case] golden T={T}")
+        g = gold["tiles"][str(T)]
+        A = B.B2SR.from_csr(csr, T, dev)
+        rp, ci, val = A.to_host()
+        assert rp.tolist() == g["rowptr"] and ci.tolist() == g["colind"]
+        assert val.astype(np.int64).tolist() == g["val"]
+        xb = B.pack_vec(B.padded_vector(np.array(gold["x_bits"], np.float32), T, dev), T)
+        assert to_np(xb, T).astype(np.int64).tolist() == g["xb"]
+        assert to_np(B.bmv_bin_bin_full(A, xb)).tolist() == g["bin_bin_full"]
+        assert to_np(B.bmv_bin_bin_bin(A, xb), T).astype(np.int64).tolist() == g["bin_bin_bin"]
+        m = np.array(g["mask"], dtype=np.uint64).astype(B.WORD_NUMPY[T]).view(to_np(xb).dtype)
+        got_m = B.bmv_bin_bin_bin_masked(A, xb, torch.from_numpy(m).to(dev))
+        assert to_np(got_m, T).astype(np.int64).tolist() == g["masked"]
+        ff = to_np(B.bmv_bin_full_full(A, B.padded_vector(np.array(gold["x_float"], np.float32), T, dev)))
+        assert ff[:n].tolist() == gold["full_full"]
+
+
+@pytest.mark.parametrize("T", TS)
+def test_random_and_ragged(orc, dev, T):
+    from bgb import graph
+    rng = np.random.default_rng(100 + T)
+    cases = [(1, 0), (2, 1), (T - 1, 3), (T + 1, 10), (5 * T + 3, 200), (1000, 3000), (4099, 20000)]
+    for n, m in cases:
+        src = rng.integers(0, n, m).astype(np.int32)
+        dst = rng.integers(0, n, m).astype(np.int32)
+        for undirected in (True, False):
+            csr = graph.from_coo(n, n, src, dst, undirected, True)
+            check_all_ops(orc, csr, T, rng, dev, f"random n={n} m={m} und={undirected}")
+
+
+@pytest.mark.parametrize("T", TS)
+def test_empty_rows_and_hubs(orc, dev, T):
+    """Long runs of empty block rows and rows spanning many chunks."""
+    from bgb import graph
+    rng = np.random.default_rng(7 * T)
+    n = 20000
+    # star: vertex 0 adjacent to every vertex -> block row 0 spans many chunks
+    src = np.zeros(n - 1, np.int32)
+    dst = np.arange(1, n, dtype=np.int32)
+    # isolated stretch [5000, 15000) and a few random edges elsewhere
+    extra_s = rng.integers(15000, n, 3000).astype(np.int32)
+    extra_d = rng.integers(15000, n, 3000).astype(np.int32)
+    s = np.concatenate([src[:4999], src[15000:], extra_s])
+    d = np.concatenate([dst[:4999], dst[15000:], extra_d])
+    csr = graph.from_coo(n, n, s, d, True, True)
+    check_all_ops(orc, csr, T, rng, dev, "star+holes", xdensity=0.5)
+    # all-empty matrix
+    csr0 = graph.from_coo(100, 100, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    check_all_ops(orc, csr0, T, rng, dev, "empty")
+    # dense block (every tile full)
+    k = 3 * T
+    ii, jj = np.meshgrid(np.arange(k), np.arange(k))
+    csr_d = graph.from_coo(k, k, ii.ravel(), jj.ravel(), False, False)
+    check_all_ops(orc, csr_d, T, rng, dev, "dense")
+
+
+@pytest.mark.parametrize("T", TS)
+def test_accumulate_semantics(orc, dev, T):
+    import torch
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(T)
+    csr = graph.rmat(12, 8, seed=5)
+    log(f"[case] accumulate T={T}")
+    A = B.B2SR.from_csr(csr, T, dev)
+    x01 = (rng.random(csr.nrows) < 0.2).astype(np.float32)
+    xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+    counts = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+    C = B.bmv_bin_bin_full(A, xb)
+    for _ in range(4):  # the drivers' 5 accumulating launches (test_bin_bin_full.cu:335-346)
+        B.bmv_bin_bin_full(A, xb, out=C, accumulate=True)
+    np.testing.assert_array_equal(to_np(C)[: csr.nrows], 5 * counts)
+    Cb = torch.zeros(A.nblockrows, dtype=B.WORD_TORCH[T], device=dev)
+    for _ in range(5):
+        B.bmv_bin_bin_bin(A, xb, out=Cb, accumulate=True)
+    np.testing.assert_array_equal(to_np(Cb, T), orc.pack_u8(T, (counts > 0).astype(np.uint8), A.nblockrows))
+    xf = rng.random(csr.nrows).astype(np.float32)
+    xfp = B.padded_vector(xf, T, dev)
+    Cf = B.bmv_bin_full_full(A, xfp)
+    B.bmv_bin_full_full(A, xfp, out=Cf, accumulate=True)
+    ref = 2 * orc.csr_f64(csr.rowptr, csr.colind, xf.astype(np.float64))
+    np.testing.assert_allclose(to_np(Cf)[: csr.nrows], ref, rtol=FP_RTOL, atol=FP_ATOL)
+
+
+def test_config_er4096_bin_bin_bin(orc, dev):
+    """ER n=4096, average degree 16, seed 1, T=32, x density 0.1: bit-exact."""
+    from bgb import graph
+    csr = graph.erdos_renyi(4096, 16, seed=1)
+    assert csr.nnz == 65536
+    check_all_ops(orc, csr, 32, np.random.default_rng(1), dev, "ER4096", xdensity=0.1)
+
+
+@pytest.mark.parametrize("T", TS)
+def test_config_rmat20_bin_bin_full(orc, dev, T):
+    """R-MAT scale 20, edge factor 16, symmetrised: exact int32 counts at every T."""
+    from bgb import b2sr as B
+    from bgb import graph
+    csr = _rmat20()
+    log(f"[case] rmat20 T={T} nnz={csr.nnz}")
+    A = B.B2SR.from_csr(csr, T, dev)
+    x01 = graph.bernoulli(csr.nrows, 0.05, 2)
+    xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+    counts = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+    got = to_np(B.bmv_bin_bin_full(A, xb))
+    np.testing.assert_array_equal(got[: csr.nrows], counts)
+    # invariant: total popcount of the packed tiles equals nnz
+    _, _, val = A.to_host()
+    assert int(np.unpackbits(val.view(np.uint8)).sum()) == csr.nnz
+
+
+_RMAT20 = None
+
+
+def _rmat20():
+    global _RMAT20
+    if _RMAT20 is None:
+        from bgb import graph
+        _RMAT20 = graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=1)
+    return _RMAT20
