@@ -137,3 +137,46 @@ def padded_vector(x, T, device="cuda"):
     out = torch.zeros(nbr * T, dtype=torch.float32, device=device)
     out[:n] = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
     return out
+
+
+def bmv_bin_full_full_bf16(A, x_bf16, out=None, accumulate=False):
+    """bin_full_full with x given as raw bfloat16 bits (torch.bfloat16 or int16)."""
+    assert x_bf16.element_size() == 2 and x_bf16.numel() >= A.nblockrows * A.T
+    C = _out(out, A.nblockrows * A.T, torch.float32, A.device, accumulate)
+    call("bgb_bmv_bin_full_full_bf16", A.T, _ptr(A.val), _ptr(x_bf16), _ptr(C), _ptr(A.rowptr),
+         _ptr(A.colind), A.nblockrows, A.nblocks, int(accumulate), _stream())
+    return C
+
+
+def bfs_multi(A, sources, use_graph=True):
+    """Batched BFS from up to 64 sources; returns (levels[nsrc, n] int32 tensor, nlevels)."""
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    n = A.nrows
+    lev = torch.empty((src.size, n), dtype=torch.int32, device=A.device)
+    nl = ctypes.c_int(0)
+    call("bgb_bfs_multi", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
+         src.ctypes.data, src.size, _ptr(lev), int(use_graph), ctypes.byref(nl), _stream())
+    return lev, nl.value
+
+
+def bfs_plain(A, source, out=None):
+    """Single-source BFS as one masked BMV per level; returns (levels[n], nlevels)."""
+    n = A.nrows
+    lev = torch.empty(n, dtype=torch.int32, device=A.device) if out is None else out
+    nl = ctypes.c_int(0)
+    call("bgb_bfs_plain", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
+         int(source), _ptr(lev), ctypes.byref(nl), _stream())
+    return lev, nl.value
+
+
+PR_MODES = {"fp32": 0, "bf16": 1, "fp64sum": 2}
+
+
+def pagerank(A, damping=0.85, iters=50, mode="fp32"):
+    """PageRank via bin_full_full; returns (pr float32 tensor [n], residuals np.float64[iters])."""
+    n = A.nrows
+    pr = torch.empty(n, dtype=torch.float32, device=A.device)
+    res = np.zeros(max(iters, 1), np.float64)
+    call("bgb_pagerank", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
+         float(damping), int(iters), PR_MODES[mode], _ptr(pr), res.ctypes.data, _stream())
+    return pr, res[:iters]

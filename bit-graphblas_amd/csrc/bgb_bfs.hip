@@ -1,0 +1,520 @@
+// Breadth-first search on a symmetric B2SR matrix (gfx950).
+//
+// The reference has no BFS; its building block is the masked bin_bin_bin BMV
+// C[I] = (A x != 0) & ~mask[I] (bsrbmv.cu:1086-1229), i.e. one BFS level of
+// one source. Two drivers are built here:
+//
+// * bgb_bfs_plain -- the per-source loop: one masked BMV per level, then an
+//   update kernel (visited |= next, level[v] = d) and a host check of the
+//   "any new vertex" flag.
+// * bgb_bfs_multi -- up to 64 sources at once: vertex v keeps one uint64
+//   frontier word F[v] (bit s = v is on source s's frontier) and one visited
+//   word V[v]. A level is one pull pass over the tiles:
+//       N[v] = OR over neighbours u with F[u] != 0 of F[u]
+//       new  = N & ~V[v];  V[v] |= new;  Fnext[v] = new
+//   Only frontier neighbours are gathered: the tile row word is first ANDed
+//   with a packed "frontier non-empty" bit vector (the B2SR vector layout),
+//   and rows whose 64 sources are all visited skip their gathers. The level
+//   of (s, v) is kept as bit-planes P[k][v] (bit s of plane k = bit k of the
+//   level), expanded to level[s*n + v] at the end. The level loop (two levels
+//   per graph, so the double buffers alternate) is captured in a hipGraph
+//   and replayed until a level finds nothing; device-side flags turn the
+//   levels after the last one into no-ops.
+// The tile walk, group scan and chunk carries are those of the BMV kernels,
+// with OR as the combine operator.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <vector>
+
+#include "bgb_device.h"
+#include "bgb_tile.h"
+
+namespace bgb {
+namespace {
+
+constexpr int kBfsWaves = 4;
+constexpr int kBfsRun = 8;
+constexpr int kPlanes = 16;  // level < 2^16
+
+__device__ __forceinline__ long long lmin2(long long a, long long b) { return a < b ? a : b; }
+
+struct BfsState {
+  const void *A;
+  const int *rowptr;
+  const int *colind;
+  int nbr, nblocks, nchunks;
+  const uint64_t *F;     // frontier of this level
+  uint64_t *Fn;          // frontier of the next level
+  const void *fnz;       // packed F != 0 (W_T words, >= ceil(NBR/T)*T)
+  void *fnzn;            // packed Fn != 0
+  uint64_t *V;           // visited
+  uint64_t *P;           // kPlanes bit-planes, stride npad
+  long long npad;        // NBR * T
+  uint64_t full;         // all source bits
+  int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done
+  int *carry_key;
+  uint64_t *carry_val;
+};
+
+// Records the vertices first reached at this level (new bits) for one row.
+__device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint64_t vold, uint64_t nb,
+                                           int depth) {
+  b.Fn[v] = nb;
+  if (nb) {
+    b.V[v] = vold | nb;
+#pragma unroll
+    for (int k = 0; k < kPlanes; ++k)
+      if ((depth >> k) & 1) b.P[k * b.npad + v] |= nb;
+    return true;
+  }
+  return false;
+}
+
+template <int T>
+__device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const uint64_t (&acc)[Tile<T>::R],
+                                          const uint64_t (&vr)[Tile<T>::R], int depth, bool &any) {
+  constexpr int R = Tile<T>::R, L = Tile<T>::L;
+  using W = typename Tile<T>::W;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const long long v = (long long)I * T + q * R + i;
+    const uint64_t nb = acc[i] & ~vr[i];
+    if (record_new(b, v, vr[i], nb, depth)) {
+      any = true;
+      bits |= 1u << (T - 1 - (q * R + i));
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
+  if (q == 0) static_cast<W *>(b.fnzn)[I] = (W)bits;
+}
+
+template <int T>
+__device__ __forceinline__ void load_visited(const BfsState &b, int I, int q, uint64_t (&vr)[Tile<T>::R]) {
+  constexpr int R = Tile<T>::R;
+#pragma unroll
+  for (int i = 0; i < R; ++i) vr[i] = b.V[(long long)I * T + q * R + i];
+}
+
+template <int T, int S>
+__global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
+  constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
+  using W = typename Tile<T>::W;
+  if (b.ctl[2]) return;  // BFS already finished: this captured level is a no-op
+  const int depth = b.ctl[0];
+  const int lane = threadIdx.x & 63;
+  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves + (threadIdx.x >> 6);
+  if (chunk >= b.nchunks) return;
+  const int g = lane / L, q = lane % L;
+  const long long k0 = (long long)chunk * CH;
+  const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
+  const long long pb = lmin2(pa + S, b.nblocks);
+  const bool active = pa < pb;
+  const int *__restrict__ rowptr = b.rowptr;
+  int I = 0;
+  if (active) {
+    int lo = 0, hi = b.nbr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rowptr[mid] <= pa) lo = mid;
+      else hi = mid - 1;
+    }
+    I = lo;
+  }
+  const int I0 = I;
+  uint32_t F[S][NW];
+  int J[S];
+  const int n = (int)(pb - pa);
+  if (active) load_run<T, S>(b.A, b.colind, pa, n, q, F, J);
+  uint32_t X[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(b.fnz)[J[s]] : 0u;
+
+  bool any = false;
+  uint64_t acc[R], zero[R], vr[R], hv[R], hvr[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) acc[i] = zero[i] = hv[i] = hvr[i] = 0;
+  if (active && pa == 0)
+    for (int r = 0; r < I0; ++r) {
+      load_visited<T>(b, r, q, vr);
+      bfs_flush<T>(b, r, q, zero, vr, depth, any);
+    }
+  if (active) load_visited<T>(b, I, q, vr);
+  int held = -1;
+  int e = active ? rowptr[I + 1] : INT_MAX;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (s < n) {
+      const uint64_t *__restrict__ fr = b.F + (long long)J[s] * T;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        uint32_t w = row_word<T>(F[s], i) & X[s];
+        if (vr[i] == b.full) w = 0;  // every source already reached this vertex
+        while (w) {
+          const int h = 31 - __clz(w);
+          acc[i] |= fr[T - 1 - h];
+          w &= ~(1u << h);
+        }
+      }
+      const int pos = (int)(pa + s) + 1;
+      if (pos == e) {
+        if (held < 0) {
+          held = I;
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            hv[i] = acc[i];
+            hvr[i] = vr[i];
+          }
+        } else {
+          bfs_flush<T>(b, I, q, acc, vr, depth, any);
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i] = 0;
+        ++I;
+        e = I < b.nbr ? rowptr[I + 1] : INT_MAX;
+        while (e == pos) {
+          load_visited<T>(b, I, q, vr);
+          bfs_flush<T>(b, I, q, zero, vr, depth, any);
+          ++I;
+          e = I < b.nbr ? rowptr[I + 1] : INT_MAX;
+        }
+        if (I < b.nbr) load_visited<T>(b, I, q, vr);
+      }
+    }
+  }
+  const int key = active ? I : INT_MAX;
+  uint64_t sc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) sc[i] = acc[i];
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int ku = __shfl_up(key, d * L);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint64_t up = __shfl_up(sc[i], d * L);
+      if (g >= d && ku == key) sc[i] |= up;
+    }
+  }
+  const int kin = __shfl_up(key, L);
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint64_t up = __shfl_up(sc[i], L);
+    if (g > 0 && kin == I0) hv[i] |= up;
+  }
+  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any);
+  const int last = (int)lmin2(G - 1, (b.nblocks - 1 - k0) / S);
+  if (g == last) {
+    const bool open = key < b.nbr;
+    if (q == 0) b.carry_key[chunk] = open ? key : -1;
+    if (open) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) b.carry_val[(long long)chunk * T + q * R + i] = sc[i];
+    }
+  }
+  if (__ballot(any) != 0 && lane == 0) b.ctl[1] = 1;
+}
+
+// Folds the carries of rows that cross chunks into this level's results.
+template <int T>
+__global__ void bfs_fixup_kernel(BfsState b) {
+  using W = typename Tile<T>::W;
+  if (b.ctl[2]) return;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)b.nchunks * T) return;
+  const long long c = tid / T;
+  const int r = (int)(tid % T);
+  const int key = b.carry_key[c];
+  if (key < 0) return;
+  const long long v = (long long)key * T + r;
+  const uint64_t cv = b.carry_val[tid];
+  if (!cv) return;
+  const uint64_t vold = b.V[v];
+  const uint64_t nb = cv & ~vold;
+  if (!nb) return;
+  const int depth = b.ctl[0];
+  atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
+  atomicOr(reinterpret_cast<unsigned long long *>(b.V + v), (unsigned long long)nb);
+  for (int k = 0; k < kPlanes; ++k)
+    if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), (unsigned long long)nb);
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(b.fnzn) + key);
+  atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3),
+           (1u << (T - 1 - r)) << ((int)(addr & 3) * 8));
+  b.ctl[1] = 1;
+}
+
+// End of a level: stop when nothing new was found, else advance the depth.
+__global__ void bfs_advance_kernel(int *ctl) {
+  if (ctl[2]) return;
+  if (ctl[1] == 0) ctl[2] = 1;
+  else ctl[0] += 1;
+  ctl[1] = 0;
+}
+
+template <class V>
+__global__ void bfs_fill_kernel(V *p, long long n, V v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Sources: V[src] and F[src] get bit s; the packed frontier word gets the bit.
+template <int T>
+__global__ void bfs_seed_kernel(const int *src, int nsrc, uint64_t *F, uint64_t *V, void *fnz) {
+  using W = typename Tile<T>::W;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int s = 0; s < nsrc; ++s) {
+    const int v = src[s];
+    F[v] |= 1ull << s;
+    V[v] |= 1ull << s;
+    static_cast<W *>(fnz)[v / T] |= (W)(1u << (T - 1 - v % T));
+  }
+}
+
+__global__ void bfs_expand_kernel(const uint64_t *V, const uint64_t *P, long long npad, int n, int nsrc,
+                                  int nplanes, int *level) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t vis = V[v];
+  uint64_t pl[kPlanes];
+#pragma unroll
+  for (int k = 0; k < kPlanes; ++k) pl[k] = k < nplanes ? P[k * npad + v] : 0;
+  for (int s = 0; s < nsrc; ++s) {
+    int lev = -1;
+    if ((vis >> s) & 1) {
+      lev = 0;
+#pragma unroll
+      for (int k = 0; k < kPlanes; ++k) lev |= (int)((pl[k] >> s) & 1) << k;
+    }
+    level[(long long)s * n + v] = lev;
+  }
+}
+
+// ---- plain per-source loop ----
+template <int T>
+__global__ void plain_update_kernel(const void *next, void *visited, int nbr, int n, int *level, int depth,
+                                    int *any) {
+  using W = typename Tile<T>::W;
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const uint32_t c = static_cast<const W *>(next)[I];
+  if (!c) return;
+  W *vis = static_cast<W *>(visited);
+  vis[I] = (W)(vis[I] | c);
+  for (int r = 0; r < T; ++r)
+    if ((c >> (T - 1 - r)) & 1u) {
+      const long long v = I * T + r;
+      if (v < n) level[v] = depth;
+    }
+  *any = 1;
+}
+
+template <int T>
+__global__ void plain_seed_kernel(int src, void *front, void *visited, int *level) {
+  using W = typename Tile<T>::W;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  static_cast<W *>(front)[src / T] = (W)(1u << (T - 1 - src % T));
+  static_cast<W *>(visited)[src / T] = (W)(1u << (T - 1 - src % T));
+  level[src] = 0;
+}
+
+unsigned nblk(long long n, int b = 256) { return (unsigned)ceil_div(n > 0 ? n : 1, b); }
+
+template <int T>
+int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
+                   const int *h_src, int nsrc, int *level, int use_graph, int *h_nlevels, hipStream_t s) {
+  constexpr int S = kBfsRun, CH = (64 / Tile<T>::L) * S;
+  const long long npad = (long long)nbr * T;
+  const long long nwords = ceil_div(nbr, T) * T;
+  const int wb = word_bytes(T);
+  const long long nchunks = ceil_div(nblocks, CH);
+  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv;
+  BGB_HIP_TRY(F0.alloc(8 * npad, s));
+  BGB_HIP_TRY(F1.alloc(8 * npad, s));
+  BGB_HIP_TRY(Vb.alloc(8 * npad, s));
+  BGB_HIP_TRY(Pb.alloc(8 * npad * kPlanes, s));
+  BGB_HIP_TRY(z0.alloc(wb * nwords + 16, s));
+  BGB_HIP_TRY(z1.alloc(wb * nwords + 16, s));
+  BGB_HIP_TRY(ctl.alloc(16, s));
+  BGB_HIP_TRY(src.alloc(sizeof(int) * nsrc, s));
+  BGB_HIP_TRY(ck.alloc(sizeof(int) * (nchunks + 1), s));
+  BGB_HIP_TRY(cv.alloc(8 * (nchunks + 1) * T, s));
+  const uint64_t full = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
+  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F0.as<uint64_t>(), npad, 0ull);
+  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F1.as<uint64_t>(), npad, 0ull);
+  bfs_fill_kernel<uint64_t><<<nblk(npad * kPlanes), 256, 0, s>>>(Pb.as<uint64_t>(), npad * kPlanes, 0ull);
+  // padding rows count as visited by every source
+  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(Vb.as<uint64_t>(), npad, 0ull);
+  if (npad > n)
+    bfs_fill_kernel<uint64_t><<<nblk(npad - n), 256, 0, s>>>(Vb.as<uint64_t>() + n, npad - n, full);
+  bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(z0.as<uint8_t>(), wb * nwords + 16, 0);
+  bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(z1.as<uint8_t>(), wb * nwords + 16, 0);
+  const int init_ctl[4] = {1, 0, 0, 0};
+  BGB_HIP_TRY(hipMemcpyAsync(ctl.ptr, init_ctl, sizeof(init_ctl), hipMemcpyHostToDevice, s));
+  BGB_HIP_TRY(hipMemcpyAsync(src.ptr, h_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, s));
+  bfs_seed_kernel<T><<<1, 64, 0, s>>>(src.as<int>(), nsrc, F0.as<uint64_t>(), Vb.as<uint64_t>(), z0.ptr);
+  BGB_HIP_TRY(hipGetLastError());
+
+  BfsState st[2];
+  for (int p = 0; p < 2; ++p) {
+    BfsState &b = st[p];
+    b.A = A;
+    b.rowptr = rowptr;
+    b.colind = colind;
+    b.nbr = nbr;
+    b.nblocks = nblocks;
+    b.nchunks = (int)nchunks;
+    b.F = p == 0 ? F0.as<uint64_t>() : F1.as<uint64_t>();
+    b.Fn = p == 0 ? F1.as<uint64_t>() : F0.as<uint64_t>();
+    b.fnz = p == 0 ? z0.ptr : z1.ptr;
+    b.fnzn = p == 0 ? z1.ptr : z0.ptr;
+    b.V = Vb.as<uint64_t>();
+    b.P = Pb.as<uint64_t>();
+    b.npad = npad;
+    b.full = full;
+    b.ctl = ctl.as<int>();
+    b.carry_key = ck.as<int>();
+    b.carry_val = cv.as<uint64_t>();
+  }
+  const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
+  const unsigned fblocks = nblk(nchunks * T);
+  auto level_pair = [&]() {
+    for (int p = 0; p < 2; ++p) {
+      if (nchunks > 0) {
+        bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
+        bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
+      }
+      bfs_advance_kernel<<<1, 1, 0, s>>>(ctl.as<int>());
+    }
+  };
+  int h_ctl[4] = {0, 0, 0, 0};
+  const int max_pairs = (1 << kPlanes) / 2 - 1;
+  if (use_graph) {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    BGB_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    level_pair();
+    BGB_HIP_TRY(hipStreamEndCapture(s, &graph));
+    BGB_HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    int pairs = 0;
+    int st_code = BGB_OK;
+    while (pairs < max_pairs) {
+      // replay a few pairs between host checks; finished levels are no-ops
+      for (int r = 0; r < 4; ++r, ++pairs)
+        if (hipGraphLaunch(exec, s) != hipSuccess) st_code = BGB_ERR_HIP;
+      if (hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st_code = BGB_ERR_HIP;
+      if (st_code || h_ctl[2]) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (st_code) return set_error(st_code, "BFS graph replay failed");
+  } else {
+    for (int pairs = 0; pairs < max_pairs; ++pairs) {
+      level_pair();
+      BGB_HIP_TRY(hipGetLastError());
+      BGB_HIP_TRY(hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, s));
+      BGB_HIP_TRY(hipStreamSynchronize(s));
+      if (h_ctl[2]) break;
+    }
+  }
+  BGB_REQUIRE(h_ctl[2], BGB_ERR_SIZE, "BFS deeper than %d levels", 1 << kPlanes);
+  const int nlevels = h_ctl[0];  // depth of the first empty level == number of levels
+  int nplanes = 0;
+  while (nplanes < kPlanes && (1 << nplanes) <= nlevels) ++nplanes;
+  bfs_expand_kernel<<<nblk(n), 256, 0, s>>>(Vb.as<uint64_t>(), Pb.as<uint64_t>(), npad, n, nsrc, nplanes,
+                                            level);
+  BGB_HIP_TRY(hipGetLastError());
+  *h_nlevels = nlevels;
+  return BGB_OK;
+}
+
+template <int T>
+int bfs_plain_impl(const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
+                   int source, int *level, int *h_nlevels, bgb_stream_t stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const long long nwords = ceil_div(nbr, T) * T;
+  const int wb = word_bytes(T);
+  DeviceBuffer f0, f1, vis, any;
+  BGB_HIP_TRY(f0.alloc(wb * nwords + 16, s));
+  BGB_HIP_TRY(f1.alloc(wb * nwords + 16, s));
+  BGB_HIP_TRY(vis.alloc(wb * nwords + 16, s));
+  BGB_HIP_TRY(any.alloc(sizeof(int), s));
+  for (DeviceBuffer *d : {&f0, &f1, &vis})
+    bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(d->as<uint8_t>(), wb * nwords + 16, 0);
+  bfs_fill_kernel<int><<<nblk(n), 256, 0, s>>>(level, n, -1);
+  plain_seed_kernel<T><<<1, 64, 0, s>>>(source, f0.ptr, vis.ptr, level);
+  BGB_HIP_TRY(hipGetLastError());
+  void *cur = f0.ptr, *nxt = f1.ptr;
+  int depth = 1;
+  for (;; ++depth) {
+    BGB_REQUIRE(depth < INT_MAX, BGB_ERR_SIZE, "BFS too deep");
+    int st = bgb_bmv_bin_bin_bin_masked(T, A, cur, nxt, rowptr, colind, nbr, nblocks, vis.ptr, stream);
+    if (st) return st;
+    bfs_fill_kernel<int><<<1, 1, 0, s>>>(any.as<int>(), 1, 0);
+    plain_update_kernel<T><<<nblk(nbr), 256, 0, s>>>(nxt, vis.ptr, nbr, n, level, depth, any.as<int>());
+    BGB_HIP_TRY(hipGetLastError());
+    int h_any = 0;
+    BGB_HIP_TRY(hipMemcpyAsync(&h_any, any.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    BGB_HIP_TRY(hipStreamSynchronize(s));
+    if (!h_any) break;
+    void *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  *h_nlevels = depth;
+  return BGB_OK;
+}
+
+int bfs_check(int T, const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
+              int *level, int *h_nlevels) {
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(rowptr && level && h_nlevels, BGB_ERR_ARG, "null argument");
+  BGB_REQUIRE(nblocks == 0 || (A && colind), BGB_ERR_ARG, "null matrix");
+  BGB_REQUIRE(n >= 1 && nbr == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");
+  auto aligned16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  BGB_REQUIRE(aligned16(A) && aligned16(colind), BGB_ERR_ARG, "bsrVal and bsrColInd must be 16-byte aligned");
+  return BGB_OK;
+}
+
+}  // namespace
+}  // namespace bgb
+
+using namespace bgb;
+
+extern "C" {
+
+int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
+                  int nblocks, int n, const int *h_sources, int nsrc, int *d_levels, int use_graph,
+                  int *h_nlevels, bgb_stream_t stream) {
+  clear_error();
+  int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_levels, h_nlevels);
+  if (st) return st;
+  BGB_REQUIRE(h_sources && nsrc >= 1 && nsrc <= 64, BGB_ERR_ARG, "need 1..64 sources");
+  for (int i = 0; i < nsrc; ++i)
+    BGB_REQUIRE(h_sources[i] >= 0 && h_sources[i] < n, BGB_ERR_ARG, "source %d out of range", h_sources[i]);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (T) {
+    case 4: return bfs_multi_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    case 8: return bfs_multi_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    case 16: return bfs_multi_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    default: return bfs_multi_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+  }
+}
+
+int bgb_bfs_plain(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
+                  int nblocks, int n, int source, int *d_level, int *h_nlevels, bgb_stream_t stream) {
+  clear_error();
+  int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_level, h_nlevels);
+  if (st) return st;
+  BGB_REQUIRE(source >= 0 && source < n, BGB_ERR_ARG, "source out of range");
+  switch (T) {
+    case 4: return bfs_plain_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+    case 8: return bfs_plain_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+    case 16: return bfs_plain_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+    default: return bfs_plain_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+  }
+}
+
+}  // extern "C"

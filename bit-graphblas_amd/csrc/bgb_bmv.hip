@@ -33,7 +33,16 @@ namespace bgb {
 
 __device__ __forceinline__ long long lmin(long long a, long long b) { return a < b ? a : b; }
 
-enum BmvOp : int { OP_BITS = 0, OP_MASKED = 1, OP_COUNT_I32 = 2, OP_COUNT_F32 = 3, OP_FULL_F32 = 4 };
+enum BmvOp : int {
+  OP_BITS = 0,
+  OP_MASKED = 1,
+  OP_COUNT_I32 = 2,
+  OP_COUNT_F32 = 3,
+  OP_FULL_F32 = 4,
+  OP_FULL_BF16 = 5  // x in bfloat16, FP32 accumulation
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 struct BmvArgs {
   const void *A;
@@ -61,6 +70,11 @@ template <>
 struct AccOf<OP_FULL_F32> {
   using type = float;
 };
+template <>
+struct AccOf<OP_FULL_BF16> {
+  using type = float;
+};
+constexpr bool is_full(int op) { return op == OP_FULL_F32 || op == OP_FULL_BF16; }
 
 template <int T, int OP>
 __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
@@ -131,7 +145,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   if (active) load_run<T, S>(a.A, a.colind, pa, n, q, F, J);
 
   uint32_t X[S];
-  if constexpr (OP != OP_FULL_F32) {
+  if constexpr (!is_full(OP)) {
 #pragma unroll
     for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
   }
@@ -161,6 +175,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
           while (w) {
             const int h = 31 - __clz(w);
             acc[i] += x[T - 1 - h];
+            w &= ~(1u << h);
+          }
+        }
+      } else if constexpr (OP == OP_FULL_BF16) {
+        const uint16_t *__restrict__ x = static_cast<const uint16_t *>(a.B) + (long long)J[s] * T;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          uint32_t w = row_word<T>(F[s], i);
+          while (w) {
+            const int h = 31 - __clz(w);
+            acc[i] += bf16_to_f32(x[T - 1 - h]);
             w &= ~(1u << h);
           }
         }
@@ -394,6 +419,13 @@ int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C, 
                           bgb_stream_t stream) {
   return bmv_entry<OP_FULL_F32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
                                 accumulate, nullptr, stream);
+}
+
+int bgb_bmv_bin_full_full_bf16(int T, const void *d_A, const uint16_t *d_B, float *d_C,
+                               const int *d_rowptr, const int *d_colind, int nblockrows,
+                               int nblocks, int accumulate, bgb_stream_t stream) {
+  return bmv_entry<OP_FULL_BF16>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
+                                 accumulate, nullptr, stream);
 }
 
 int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_C,

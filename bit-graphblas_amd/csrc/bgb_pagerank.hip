@@ -1,0 +1,202 @@
+// PageRank over a symmetric B2SR matrix using the bin_full_full BMV (gfx950).
+//
+// No reference counterpart: the reference's bin_full_full BMV
+// (bsrbmv.cu:826-1081) is the building block. Iteration (pr_0 = 1/n):
+//   x_t[v]   = pr_t[v] / deg(v)            (0 for deg(v) = 0)
+//   y        = A x_t                        (bin_full_full, FP32 accumulate)
+//   pr_t+1   = (1-d)/n + d * (y + dangling_t / n),   dangling_t = sum_{deg=0} pr_t
+//   res_t    = || pr_t+1 - pr_t ||_1
+// Precision modes:
+//   0  FP32 vector, FP32 BMV, FP32 residual / dangling sums
+//   1  x stored as bfloat16 (round to nearest even), FP32 accumulation and
+//      sums -- halves the gathered bytes of x
+//   2  FP32 vector and BMV, residual and dangling / teleport sums in FP64
+// The update kernel fuses the next x, the dangling sum and the residual;
+// reductions are two-level (per-block partials, one finishing block) so the
+// results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include "bgb_device.h"
+
+extern "C" int bgb_bmv_bin_full_full_bf16(int T, const void *d_A, const uint16_t *d_B, float *d_C,
+                                          const int *d_rowptr, const int *d_colind, int nblockrows,
+                                          int nblocks, int accumulate, bgb_stream_t stream);
+
+namespace bgb {
+namespace {
+
+constexpr int kPrBlock = 256;
+
+// float -> bfloat16 with round-to-nearest-even on the upper 16 bits.
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <class Acc>
+__device__ __forceinline__ Acc block_sum(Acc v, Acc *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  Acc t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kPrBlock / 64; ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+// x from pr, dangling partials; used at start and after each update.
+template <class Acc, bool BF>
+__global__ __launch_bounds__(kPrBlock) void pr_init_kernel(float *pr, const int *deg, int n, long long npad,
+                                                           void *x, Acc *part_dang) {
+  __shared__ Acc sh[kPrBlock / 64];
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  Acc dang = 0;
+  if (v < npad) {
+    float p = v < n ? 1.0f / (float)n : 0.0f;
+    if (v < n) pr[v] = p;
+    const int dg = v < n ? deg[v] : 0;
+    const float xv = dg > 0 ? p / (float)dg : 0.0f;
+    if (BF) static_cast<uint16_t *>(x)[v] = f32_to_bf16(xv);
+    else static_cast<float *>(x)[v] = xv;
+    if (v < n && dg == 0) dang = (Acc)p;
+  }
+  const Acc t = block_sum<Acc>(dang, sh);
+  if (threadIdx.x == 0) part_dang[blockIdx.x] = t;
+}
+
+template <class Acc, bool BF>
+__global__ __launch_bounds__(kPrBlock) void pr_update_kernel(const float *y, const float *pr, float *pr_next,
+                                                             const int *deg, int n, long long npad, float damping,
+                                                             const Acc *dang_total, void *x, Acc *part_res,
+                                                             Acc *part_dang) {
+  __shared__ Acc sh[kPrBlock / 64];
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  Acc res = 0, dang = 0;
+  if (v < npad) {
+    float xv = 0.0f;
+    if (v < n) {
+      const Acc tele = (Acc)(1.0f - damping) / (Acc)n;
+      const Acc pn = tele + (Acc)damping * ((Acc)y[v] + *dang_total / (Acc)n);
+      const Acc po = (Acc)pr[v];
+      res = pn > po ? pn - po : po - pn;
+      const float p = (float)pn;
+      pr_next[v] = p;
+      const int dg = deg[v];
+      xv = dg > 0 ? p / (float)dg : 0.0f;
+      if (dg == 0) dang = (Acc)p;
+    }
+    if (BF) static_cast<uint16_t *>(x)[v] = f32_to_bf16(xv);
+    else static_cast<float *>(x)[v] = xv;
+  }
+  const Acc r = block_sum<Acc>(res, sh);
+  const Acc d = block_sum<Acc>(dang, sh);
+  if (threadIdx.x == 0) {
+    part_res[blockIdx.x] = r;
+    part_dang[blockIdx.x] = d;
+  }
+}
+
+// One block folds the partials: residual -> res_out[it] (as double), dangling -> dang_total.
+template <class Acc>
+__global__ __launch_bounds__(kPrBlock) void pr_finish_kernel(const Acc *part_res, const Acc *part_dang, int nparts,
+                                                             double *res_out, Acc *dang_total) {
+  __shared__ Acc sh[kPrBlock / 64];
+  Acc r = 0, d = 0;
+  for (int i = threadIdx.x; i < nparts; i += kPrBlock) {
+    if (part_res) r += part_res[i];
+    d += part_dang[i];
+  }
+  r = block_sum<Acc>(r, sh);
+  d = block_sum<Acc>(d, sh);
+  if (threadIdx.x == 0) {
+    if (res_out) *res_out = (double)r;
+    *dang_total = d;
+  }
+}
+
+// deg(v) = popcount of row v over its block row's tiles.
+template <int T>
+__global__ void row_degree_kernel(const void *A, const int *rowptr, int nbr, int *deg) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (long long)nbr * T) return;
+  const int I = (int)(v / T), r = (int)(v % T);
+  int d = 0;
+  for (int k = rowptr[I]; k < rowptr[I + 1]; ++k) {
+    uint32_t w;
+    if (T == 32) w = static_cast<const uint32_t *>(A)[(long long)k * T + r];
+    else if (T == 16) w = static_cast<const uint16_t *>(A)[(long long)k * T + r];
+    else w = static_cast<const uint8_t *>(A)[(long long)k * T + r];
+    d += __popc(w);
+  }
+  deg[v] = d;
+}
+
+template <class Acc, bool BF>
+int pagerank_impl(int T, const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
+                  float damping, int iters, float *pr_out, double *h_res, hipStream_t s) {
+  const long long npad = (long long)nbr * T;
+  const unsigned blocks = (unsigned)ceil_div(npad, kPrBlock);
+  DeviceBuffer deg, x, y, pr2, pres, pdang, dtot, res;
+  BGB_HIP_TRY(deg.alloc(sizeof(int) * npad, s));
+  BGB_HIP_TRY(x.alloc((BF ? 2 : 4) * npad + 16, s));
+  BGB_HIP_TRY(y.alloc(sizeof(float) * npad, s));
+  BGB_HIP_TRY(pr2.alloc(sizeof(float) * n, s));
+  BGB_HIP_TRY(pres.alloc(sizeof(Acc) * blocks, s));
+  BGB_HIP_TRY(pdang.alloc(sizeof(Acc) * blocks, s));
+  BGB_HIP_TRY(dtot.alloc(sizeof(Acc), s));
+  BGB_HIP_TRY(res.alloc(sizeof(double) * (iters > 0 ? iters : 1), s));
+  switch (T) {
+    case 4: row_degree_kernel<4><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 8: row_degree_kernel<8><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 16: row_degree_kernel<16><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    default: row_degree_kernel<32><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+  }
+  pr_init_kernel<Acc, BF><<<blocks, kPrBlock, 0, s>>>(pr_out, deg.as<int>(), n, npad, x.ptr, pdang.as<Acc>());
+  pr_finish_kernel<Acc><<<1, kPrBlock, 0, s>>>(nullptr, pdang.as<Acc>(), (int)blocks, nullptr, dtot.as<Acc>());
+  BGB_HIP_TRY(hipGetLastError());
+  float *cur = pr_out, *nxt = pr2.as<float>();
+  for (int it = 0; it < iters; ++it) {
+    int st = BF ? bgb_bmv_bin_full_full_bf16(T, A, x.as<uint16_t>(), y.as<float>(), rowptr, colind, nbr,
+                                             nblocks, 0, s)
+                : bgb_bmv_bin_full_full(T, A, x.as<float>(), y.as<float>(), rowptr, colind, nbr, nblocks, 0, s);
+    if (st) return st;
+    pr_update_kernel<Acc, BF><<<blocks, kPrBlock, 0, s>>>(y.as<float>(), cur, nxt, deg.as<int>(), n, npad, damping,
+                                                         dtot.as<Acc>(), x.ptr, pres.as<Acc>(), pdang.as<Acc>());
+    pr_finish_kernel<Acc><<<1, kPrBlock, 0, s>>>(pres.as<Acc>(), pdang.as<Acc>(), (int)blocks,
+                                                 res.as<double>() + it, dtot.as<Acc>());
+    BGB_HIP_TRY(hipGetLastError());
+    float *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != pr_out)
+    BGB_HIP_TRY(hipMemcpyAsync(pr_out, cur, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+  if (h_res && iters > 0)
+    BGB_HIP_TRY(hipMemcpyAsync(h_res, res.ptr, sizeof(double) * iters, hipMemcpyDeviceToHost, s));
+  BGB_HIP_TRY(hipStreamSynchronize(s));
+  return BGB_OK;
+}
+
+}  // namespace
+}  // namespace bgb
+
+using namespace bgb;
+
+extern "C" int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
+                            int nblocks, int n, float damping, int iters, int mode, float *d_pr,
+                            double *h_residuals, bgb_stream_t stream) {
+  clear_error();
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(d_rowptr && d_pr && (nblocks == 0 || (d_A && d_colind)), BGB_ERR_ARG, "null argument");
+  BGB_REQUIRE(n >= 1 && nblockrows == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");
+  BGB_REQUIRE(iters >= 0 && damping >= 0.0f && damping <= 1.0f, BGB_ERR_ARG, "bad iters / damping");
+  BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "mode must be 0 (fp32), 1 (bf16 x) or 2 (fp64 sums)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (mode) {
+    case 0: return pagerank_impl<float, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+    case 1: return pagerank_impl<float, true>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+    default: return pagerank_impl<double, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+  }
+}

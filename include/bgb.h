@@ -116,6 +116,13 @@ int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C,
                           const int *d_rowptr, const int *d_colind, int nblockrows,
                           int nblocks, int accumulate, bgb_stream_t stream);
 
+/* bin_full_full with x stored as bfloat16 (raw uint16 bits), FP32
+ * accumulation. No reference counterpart (the reference's x is FP32); used by
+ * the BF16 PageRank variant to halve the gathered bytes of x. */
+int bgb_bmv_bin_full_full_bf16(int T, const void *d_A, const uint16_t *d_B, float *d_C,
+                               const int *d_rowptr, const int *d_colind, int nblockrows,
+                               int nblocks, int accumulate, bgb_stream_t stream);
+
 /* C[I] = bits & ~mask[I] (always overwrites).  bmv{T}_bin_bin_bin_masked
  * (bsrbmv.cu:1086,1122,1158,1194), with the reference's T=8/16 shift and
  * nibble-mask defects corrected (the mask applies to all T bits). */
@@ -123,6 +130,37 @@ int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_
                                const int *d_rowptr, const int *d_colind,
                                int nblockrows, int nblocks, const void *d_mask,
                                bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * BFS on a symmetric B2SR matrix (no reference counterpart; the reference's *
+ * building block is the masked BMV above). n = number of vertices,         *
+ * nblockrows = ceil(n/T). Levels are int32 hop counts, -1 = unreached.     *
+ * ------------------------------------------------------------------------ */
+
+/* Up to 64 sources at once: one uint64 frontier word per vertex, one pull
+ * pass over the tiles per level. d_levels[s*n + v] = level of v from
+ * h_sources[s]. use_graph = 1 captures the level loop in a hipGraph.
+ * *h_nlevels = number of non-empty levels (max level + 1). Synchronises. */
+int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
+                  int nblockrows, int nblocks, int n, const int *h_sources, int nsrc,
+                  int *d_levels, int use_graph, int *h_nlevels, bgb_stream_t stream);
+
+/* One source: one masked bin_bin_bin BMV per level (frontier & ~visited),
+ * the per-source loop the batched BFS is compared with. Synchronises once
+ * per level. */
+int bgb_bfs_plain(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
+                  int nblockrows, int nblocks, int n, int source, int *d_level,
+                  int *h_nlevels, bgb_stream_t stream);
+
+/* PageRank with bin_full_full (no reference counterpart): pr_0 = 1/n,
+ * pr_{t+1} = (1-d)/n + d*(A (pr_t/deg) + dangling_t/n), `iters` iterations.
+ * mode 0: FP32 throughout; 1: x = pr/deg stored as bfloat16, FP32 accumulate;
+ * 2: FP32 vector, FP64 residual and dangling/teleport sums. d_pr[n] receives
+ * the result; h_residuals[iters] (may be NULL) the L1 change per iteration.
+ * Synchronises. */
+int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
+                 int nblockrows, int nblocks, int n, float damping, int iters, int mode,
+                 float *d_pr, double *h_residuals, bgb_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
  * Host graph generators and CSR normalisation (no reference counterpart    *
