@@ -113,16 +113,9 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   const long long pb = lmin2(pa + S, b.nblocks);
   const bool active = pa < pb;
   const int *__restrict__ rowptr = b.rowptr;
-  int I = 0;
-  if (active) {
-    int lo = 0, hi = b.nbr - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (rowptr[mid] <= pa) lo = mid;
-      else hi = mid - 1;
-    }
-    I = lo;
-  }
+  const int Ia = wave_find_row(rowptr, 0, b.nbr - 1, k0, lane);
+  const int Ib = wave_find_row(rowptr, Ia, b.nbr - 1, lmin2(k0 + CH, b.nblocks) - 1, lane);
+  int I = active ? group_find_row(rowptr, Ia, Ib, pa) : 0;
   const int I0 = I;
   uint32_t F[S][NW];
   int J[S];

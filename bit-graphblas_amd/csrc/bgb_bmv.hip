@@ -127,16 +127,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   const int *__restrict__ rowptr = a.rowptr;
 
   // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
-  int I = 0;
-  if (active) {
-    int lo = 0, hi = a.nbr - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (rowptr[mid] <= pa) lo = mid;
-      else hi = mid - 1;
-    }
-    I = lo;
-  }
+  const int Ia = wave_find_row(rowptr, 0, a.nbr - 1, k0, lane);
+  const int Ib = wave_find_row(rowptr, Ia, a.nbr - 1, lmin(k0 + CH, a.nblocks) - 1, lane);
+  int I = active ? group_find_row(rowptr, Ia, Ib, pa) : 0;
   const int I0 = I;
 
   uint32_t F[S][NW];

@@ -125,6 +125,34 @@ __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *
   }
 }
 
+// Largest I in [lo, hi] with rowptr[I] <= p, given rowptr[lo] <= p, searched
+// by the whole wave: each step probes 64 evenly spaced entries and keeps the
+// interval between the last probe <= p and the next one (log64 steps). Must
+// be called by all 64 lanes.
+__device__ __forceinline__ int wave_find_row(const int *__restrict__ rowptr, int lo, int hi, long long p,
+                                             int lane) {
+  while (hi > lo) {
+    const long long span = hi - lo;
+    const int idx = lo + (int)((span * (lane + 1)) >> 6);
+    const int cnt = __popcll(__ballot(rowptr[idx] <= p));
+    const int nlo = cnt > 0 ? lo + (int)((span * cnt) >> 6) : lo;
+    const int nhi = cnt < 64 ? lo + (int)((span * (cnt + 1)) >> 6) - 1 : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo;
+}
+
+// Row of tile pa for one lane group, searched inside the wave's row range [lo, hi].
+__device__ __forceinline__ int group_find_row(const int *__restrict__ rowptr, int lo, int hi, long long pa) {
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rowptr[mid] <= pa) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Workgroup index remapped so that consecutive index ranges stay on one XCD
 // (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
 __device__ __forceinline__ int xcd_remap(int b, int n) {
