@@ -98,6 +98,37 @@ def cpu_baseline(orc, csr, x01, budget_s=8.0):
     return y, t, reps
 
 
+def rocsparse_baseline(torch, csr, x01, reps=20):
+    """rocSPARSE csrmv (CSR FP32, alpha=1, beta=0, x as 0/1 floats) on the same graph."""
+    import ctypes
+    path = os.path.join(REPO, "bit-graphblas_amd", "lib", "libbgb_baseline.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    f = L.bgbx_rocsparse_csrmv
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    f.restype = ci
+    f.argtypes = [ci, ci, vp, vp, vp, vp, vp, ci, ci, vp, vp, vp]
+    dev = torch.device("cuda:0")
+    rp = torch.from_numpy(csr.rowptr).to(dev)
+    cl = torch.from_numpy(csr.colind).to(dev)
+    val = torch.ones(csr.nnz, dtype=torch.float32, device=dev)
+    x = torch.from_numpy(x01).to(dev)
+    y = torch.zeros(csr.nrows, dtype=torch.float32, device=dev)
+    out = {}
+    for name, alg in (("adaptive", 0), ("rowsplit", 1)):
+        ms, pre = ctypes.c_double(), ctypes.c_double()
+        st = f(csr.nrows, csr.nnz, rp.data_ptr(), cl.data_ptr(), val.data_ptr(), x.data_ptr(), y.data_ptr(),
+               alg, reps, ctypes.byref(ms), ctypes.byref(pre), torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            out[name] = {"error": st}
+            continue
+        out[name] = {"ms": round(ms.value, 5), "gteps": round(csr.nnz / (ms.value / 1e3) / 1e9, 3),
+                     "preprocess_ms": round(pre.value, 3)}
+    del rp, cl, val, x, y
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +184,9 @@ def main():
     ceiling = copy_ceiling_gbs(torch)
     log(f"[bench] copy ceiling {ceiling:.0f} GB/s")
 
+    rs = rocsparse_baseline(torch, csr, x01)
+    log(f"[bench] rocSPARSE csrmv: {rs}")
+
     cpu = None
     if not args.no_cpu:
         import oracle_lib
@@ -188,6 +222,7 @@ def main():
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4), "traffic": None},
         "cpu_baseline": cpu,
         "copy_ceiling_gbs": round(ceiling, 1),
+        "rocsparse_csrmv": rs,
         "per_tile": {str(k): {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                               for kk, vv in v.items()} for k, v in per_tile.items()},
     }

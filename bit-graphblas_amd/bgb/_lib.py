@@ -47,9 +47,14 @@ SIGNATURES = {
     "bgb_bmv_bin_full_full_bf16": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_pagerank": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, ctypes.c_float, _c_int, _c_int,
                               _vp, _vp, _vp]),
+    "bgb_k16_pack_rhs": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "bgb_bmv_bin_full_full_k16": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
+    "bgb_k16_batched": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "bgb_bfs_multi": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int,
                                _vp, _vp]),
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "bgb_autotune_tiledim": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp]),
     "bgb_csr_from_coo": (_c_int, [_c_int, _c_int, _c_ll, _vp, _vp, _c_int, _c_int, ctypes.POINTER(_vp)]),
     "bgb_gen_er": (_c_int, [_c_int, _c_ll, _c_u64, ctypes.POINTER(_vp)]),
     "bgb_gen_rmat": (_c_int, [_c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_u64, ctypes.POINTER(_vp)]),

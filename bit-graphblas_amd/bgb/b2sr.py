@@ -180,3 +180,81 @@ def pagerank(A, damping=0.85, iters=50, mode="fp32"):
     call("bgb_pagerank", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
          float(damping), int(iters), PR_MODES[mode], _ptr(pr), res.ctypes.data, _stream())
     return pr, res[:iters]
+
+
+def k16_pack_rhs(X, nrows_pad=None):
+    """X: torch.float16 [n, 16] on the device -> MFMA-ordered RHS for the K=16 kernels.
+
+    nrows_pad (e.g. NBR*T of the matrix) sets how many rows the packed form covers."""
+    assert X.dtype == torch.float16 and X.dim() == 2 and X.shape[1] == 16 and X.is_contiguous()
+    n = X.shape[0]
+    rows = max(n, nrows_pad or 0)
+    Xb = torch.empty(-(-rows // 16) * 256, dtype=torch.float16, device=X.device)
+    if rows > n:
+        Xp = torch.zeros((rows, 16), dtype=torch.float16, device=X.device)
+        Xp[:n] = X
+        X, n = Xp, rows
+    call("bgb_k16_pack_rhs", _ptr(X), n, _ptr(Xb), _stream())
+    return Xb
+
+
+def bmv_k16(A, Xb, out=None, accumulate=False):
+    """Y[NBR*T, 16] (+)= A X with FP16 X (packed by k16_pack_rhs) on MFMA, FP32 accumulate."""
+    assert A.T in (16, 32)
+    n = A.nblockrows * A.T
+    Y = _out(out, n * 16, torch.float32, A.device, accumulate).view(-1)
+    call("bgb_bmv_bin_full_full_k16", A.T, _ptr(A.val), _ptr(Xb), _ptr(Y), _ptr(A.rowptr), _ptr(A.colind),
+         A.nblockrows, A.nblocks, int(accumulate), _stream())
+    return Y.view(n, 16)
+
+
+class K16Graph(ctypes.Structure):
+    _fields_ = [("T", ctypes.c_int), ("n", ctypes.c_int), ("nblocks", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("rowptr_off", ctypes.c_longlong), ("tile_off", ctypes.c_longlong),
+                ("val_byte_off", ctypes.c_longlong), ("x_off", ctypes.c_longlong), ("y_off", ctypes.c_longlong)]
+
+
+class K16Batch:
+    """Many (B2SR, packed RHS) pairs concatenated for one persistent K=16 launch."""
+
+    def __init__(self, mats, xbs, nworkgroups=0):
+        assert len(mats) == len(xbs) and mats
+        dev = mats[0].device
+        descs = (K16Graph * len(mats))()
+        rps, cis, vals, xs = [], [], [], []
+        rp_off = ci_off = v_off = x_off = y_off = 0
+        self.y_offsets = []
+        for i, (A, Xb) in enumerate(zip(mats, xbs)):
+            d = descs[i]
+            d.T, d.n, d.nblocks = A.T, A.nrows, A.nblocks
+            d.rowptr_off, d.tile_off, d.val_byte_off, d.x_off, d.y_off = rp_off, ci_off, v_off, x_off, y_off
+            rps.append(A.rowptr)
+            cis.append(A.colind[: max(A.nblocks, 1)])
+            vb = A.val[: max(A.nblocks, 1) * A.T].view(torch.uint8)
+            pad = (-vb.numel()) % 16
+            vals.append(torch.cat([vb, torch.zeros(pad, dtype=torch.uint8, device=dev)]) if pad else vb)
+            need = -(-(A.nblockrows * A.T) // 16) * 256
+            assert Xb.numel() >= need
+            xs.append(Xb[:need])
+            self.y_offsets.append((y_off, A.nblockrows * A.T))
+            rp_off += A.rowptr.numel()
+            ci_off += cis[-1].numel()
+            v_off += vals[-1].numel()
+            x_off += need
+            y_off += A.nblockrows * A.T * 16
+        self.descs = descs
+        self.rowptr = torch.cat(rps)
+        self.colind = torch.cat(cis)
+        self.val = torch.cat(vals)
+        self.xb = torch.cat(xs)
+        self.y = torch.empty(y_off, dtype=torch.float32, device=dev)
+        self.nworkgroups = nworkgroups
+
+    def run(self):
+        call("bgb_k16_batched", len(self.descs), ctypes.addressof(self.descs), _ptr(self.rowptr),
+             _ptr(self.colind), _ptr(self.val), _ptr(self.xb), _ptr(self.y), int(self.nworkgroups), _stream())
+        return self.y
+
+    def result(self, i):
+        off, rows = self.y_offsets[i]
+        return self.y[off: off + rows * 16].view(rows, 16)

@@ -78,3 +78,26 @@ def uniform(n, lo, hi, seed):
     out = np.empty(n, np.float32)
     call("bgb_gen_uniform_f32", n, lo, hi, seed, out.ctypes.data)
     return out
+
+
+def tile_stats(csr):
+    """Exact tile counts (and non-empty block rows) for T = 4, 8, 16, 32."""
+    nb = np.zeros(4, np.int64)
+    ne = np.zeros(4, np.int64)
+    call("bgb_tile_stats", csr.nrows, csr.ncols, csr.rowptr.ctypes.data,
+         csr.colind.ctypes.data if csr.nnz else None, nb.ctypes.data, ne.ctypes.data)
+    return {T: int(nb[i]) for i, T in enumerate((4, 8, 16, 32))}, {T: int(ne[i]) for i, T in enumerate((4, 8, 16, 32))}
+
+
+OP_FAMILY = {"bin_bin": 0, "bin_full_full": 1, "k16": 2}
+
+
+def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):
+    """(best T, predicted microseconds per T) from the tile statistics and the cost model."""
+    nbd, _ = stats if stats is not None else tile_stats(csr)
+    nb = np.array([nbd[T] for T in (4, 8, 16, 32)], np.int64)
+    cost = np.zeros(4, np.float64)
+    mask = sum(1 << i for i, T in enumerate((4, 8, 16, 32)) if T in allowed)
+    t = ctypes.c_int(0)
+    call("bgb_autotune_tiledim", OP_FAMILY[op], csr.nrows, nb.ctypes.data, mask, ctypes.byref(t), cost.ctypes.data)
+    return t.value, {T: float(cost[i]) for i, T in enumerate((4, 8, 16, 32))}

@@ -1,0 +1,351 @@
+// bin_full_full with K = 16 right-hand sides on the MFMA units (gfx950).
+//
+// No reference counterpart (the reference's bin_full_full takes one FP32
+// vector, bsrbmv.cu:826-1081). Y[v][0..16) (+)= sum_u A[v][u] X[u][0..16):
+// the 0/1 tile is exact in FP16, X is FP16, accumulation is FP32.
+//
+// One v_mfma_f32_16x16x16_f16 per 16x16 sub-tile:
+//   A operand  lane l: row l%16, columns 4*(l/16)..+3 of the tile -> 4 bits of
+//              the row word expanded to FP16 0/1
+//   B operand  lane l: RHS column l%16, tile columns 4*(l/16)..+3 -> 4 FP16
+//              values of X; X is stored pre-permuted in exactly this order
+//              (bgb_k16_pack_rhs), so each 16-row block of X is one contiguous
+//              512-byte read per wave
+//   D          lane l: rows 4*(l/16)..+3, RHS column l%16 (FP32)
+// T = 16 is one MFMA per tile; T = 32 is four (two row halves x two column
+// halves). Work decomposition follows the BMV kernels: fixed chunks of tiles
+// per wave, rows flushed when they end, a carry record for the row open at a
+// chunk's end, and a fix-up pass that adds the records with FP32 atomics.
+// The batched entry point runs many graphs in one persistent launch whose
+// waves take chunks from a global queue until it is drained.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <vector>
+
+#include "bgb_device.h"
+#include "bgb_tile.h"
+
+namespace bgb {
+namespace {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kK = 16;        // right-hand sides
+constexpr int kChunk = 32;    // tiles per wave
+constexpr int kWaves = 4;     // waves per workgroup
+constexpr int kUnroll = 4;    // tiles loaded ahead
+
+// 4 bits (bit 3 = first column) -> four FP16 values 0.0 / 1.0.
+__device__ __forceinline__ half4_t nib_to_h4(uint32_t nib) {
+  const uint32_t one = 0x3C00u;
+  uint2 u;
+  u.x = ((nib >> 3) & 1u) * one | (((nib >> 2) & 1u) * one) << 16;
+  u.y = ((nib >> 1) & 1u) * one | ((nib & 1u) * one) << 16;
+  return __builtin_bit_cast(half4_t, u);
+}
+
+struct K16Graph {  // device-side view of one graph
+  const int *rowptr;
+  const int *colind;
+  const void *val;
+  const uint2 *xb;  // 4 FP16 per lane per 16-row block
+  float *y;
+  int nbr, nblocks, accumulate;
+};
+
+template <int T>
+__device__ __forceinline__ void k16_flush(const K16Graph &G, int I, int lane, const float4_t (&D)[T / 16]) {
+  constexpr int MH = T / 16;
+  const int n = lane & 15, rq = lane >> 4;
+#pragma unroll
+  for (int mh = 0; mh < MH; ++mh)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float *p = G.y + ((long long)I * T + mh * 16 + 4 * rq + j) * kK + n;
+      *p = G.accumulate ? *p + D[mh][j] : D[mh][j];
+    }
+}
+
+// Processes tiles [k0, k1) of one graph; returns the row left open (or nbr).
+template <int T>
+__device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane, float *carry) {
+  constexpr int MH = T / 16;
+  using W = typename Tile<T>::W;
+  const int *__restrict__ rowptr = G.rowptr;
+  const W *__restrict__ A = static_cast<const W *>(G.val);
+  const int m = lane & 15, kq = lane >> 4;
+  int I = wave_find_row(rowptr, 0, G.nbr - 1, k0, lane);
+  float4_t D[MH], Z[MH];
+#pragma unroll
+  for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh] = float4_t{0.f, 0.f, 0.f, 0.f};
+  if (k0 == 0)
+    for (int r = 0; r < I; ++r) k16_flush<T>(G, r, lane, Z);
+  int e = rowptr[I + 1];
+  for (long long p0 = k0; p0 < k1; p0 += kUnroll) {
+    uint32_t w[kUnroll][MH];
+    uint2 xv[kUnroll][MH];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const long long p = p0 + u;
+      if (p < k1) {
+        const int J = G.colind[p];
+#pragma unroll
+        for (int h = 0; h < MH; ++h) {
+          w[u][h] = A[p * T + h * 16 + m];
+          xv[u][h] = G.xb[((long long)J * MH + h) * 64 + lane];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const long long p = p0 + u;
+      if (p < k1) {
+#pragma unroll
+        for (int kh = 0; kh < MH; ++kh) {
+          const half4_t b = __builtin_bit_cast(half4_t, xv[u][kh]);
+#pragma unroll
+          for (int mh = 0; mh < MH; ++mh) {
+            const uint32_t nib = (w[u][mh] >> (T - 4 - (kh * 16 + 4 * kq))) & 0xFu;
+            D[mh] = __builtin_amdgcn_mfma_f32_16x16x16f16(nib_to_h4(nib), b, D[mh], 0, 0, 0);
+          }
+        }
+        const int pos = (int)p + 1;
+        if (pos == e) {
+          k16_flush<T>(G, I, lane, D);
+#pragma unroll
+          for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh];
+          ++I;
+          e = I < G.nbr ? rowptr[I + 1] : INT_MAX;
+          while (e == pos) {
+            k16_flush<T>(G, I, lane, Z);
+            ++I;
+            e = I < G.nbr ? rowptr[I + 1] : INT_MAX;
+          }
+        }
+      }
+    }
+  }
+  if (I < G.nbr) {
+    const int rq = lane >> 4;
+#pragma unroll
+    for (int mh = 0; mh < MH; ++mh)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) carry[(mh * 16 + 4 * rq + j) * kK + m] = D[mh][j];
+  }
+  return I;
+}
+
+struct K16Args {
+  K16Graph g;
+  int nchunks;
+  int *carry_key;
+  float *carry_val;  // [nchunks][32 * 16]
+};
+
+template <int T>
+__global__ __launch_bounds__(64 * kWaves) void k16_kernel(K16Args a) {
+  const int lane = threadIdx.x & 63;
+  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+  if (chunk >= a.nchunks) return;
+  const long long k0 = (long long)chunk * kChunk;
+  const long long k1 = k0 + kChunk < a.g.nblocks ? k0 + kChunk : a.g.nblocks;
+  const int open = k16_chunk<T>(a.g, k0, k1, lane, a.carry_val + (long long)chunk * 32 * kK);
+  if (lane == 0) a.carry_key[chunk] = open < a.g.nbr ? open : -1;
+}
+
+// carry records -> Y (FP32 atomics); `graph_of` maps a chunk to its graph
+// (nullptr for a single graph).
+__global__ void k16_fixup_kernel(const int *carry_key, const float *carry_val, long long nchunks,
+                                 const K16Graph *graphs, const int *graph_of, const int *tdim, K16Graph single,
+                                 int single_T) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nchunks * 32 * kK) return;
+  const long long c = tid / (32 * kK);
+  const int e = (int)(tid % (32 * kK));
+  const int key = carry_key[c];
+  if (key < 0) return;
+  const int gi = graph_of ? graph_of[c] : 0;
+  const int T = graph_of ? tdim[gi] : single_T;
+  const int row = e / kK, col = e % kK;
+  if (row >= T) return;
+  const float v = carry_val[tid];
+  if (v == 0.0f) return;
+  float *y = graph_of ? graphs[gi].y : single.y;
+  unsafeAtomicAdd(y + ((long long)key * T + row) * kK + col, v);
+}
+
+// Persistent launch over many graphs: waves pull chunk indices from a queue.
+struct K16BatchArgs {
+  const K16Graph *graphs;
+  const int *tdim;
+  const long long *chunk_start;  // [ngraphs + 1] prefix of chunk counts
+  int ngraphs;
+  long long total;
+  unsigned long long *queue;  // next chunk index
+  int *carry_key;
+  float *carry_val;
+  int *graph_of;
+};
+
+__global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArgs a) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned long long c = 0;
+    if (lane == 0) c = atomicAdd(a.queue, 1ull);
+    c = __shfl(c, 0);
+    if (c >= (unsigned long long)a.total) break;  // uniform per wave: every wave exits once drained
+    int lo = 0, hi = a.ngraphs - 1;  // graph with chunk_start[g] <= c < chunk_start[g+1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.chunk_start[mid] <= (long long)c) lo = mid;
+      else hi = mid - 1;
+    }
+    const K16Graph G = a.graphs[lo];
+    const long long k0 = ((long long)c - a.chunk_start[lo]) * kChunk;
+    const long long k1 = k0 + kChunk < G.nblocks ? k0 + kChunk : G.nblocks;
+    float *carry = a.carry_val + (long long)c * 32 * kK;
+    const int open = a.tdim[lo] == 32 ? k16_chunk<32>(G, k0, k1, lane, carry)
+                                      : k16_chunk<16>(G, k0, k1, lane, carry);
+    if (lane == 0) {
+      a.carry_key[c] = open < G.nbr ? open : -1;
+      a.graph_of[c] = lo;
+    }
+  }
+}
+
+// Xb[(b*64 + l)*4 + j] = X[(16b + 4(l/16) + j)*16 + l%16], zero beyond n.
+__global__ void k16_pack_rhs_kernel(const uint16_t *X, long long n, long long nblk16, uint16_t *Xb) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk16 * 256) return;
+  const long long b = t / 256;
+  const int r = (int)(t % 256), l = r / 4, j = r % 4;
+  const long long v = 16 * b + 4 * (l / 16) + j;
+  Xb[t] = v < n ? X[v * kK + (l % 16)] : (uint16_t)0;
+}
+
+template <class V>
+__global__ void k16_fill_kernel(V *p, long long n, V v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+unsigned nb256(long long n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
+
+int k16_check(int T, const void *A, const void *Xb, const float *Y, const int *rowptr, const int *colind, int nbr,
+              int nblocks) {
+  BGB_REQUIRE(T == 16 || T == 32, BGB_ERR_TILEDIM, "K=16 MFMA path supports tile dims 16 and 32 (got %d)", T);
+  BGB_REQUIRE(Y && rowptr && nbr >= 0 && nblocks >= 0, BGB_ERR_ARG, "bad output / sizes");
+  BGB_REQUIRE(nblocks == 0 || (A && Xb && colind), BGB_ERR_ARG, "null matrix / RHS");
+  BGB_REQUIRE(((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(Xb)) & 15) == 0, BGB_ERR_ARG,
+              "bsrVal and the packed RHS must be 16-byte aligned");
+  return BGB_OK;
+}
+
+}  // namespace
+}  // namespace bgb
+
+using namespace bgb;
+
+extern "C" {
+
+int bgb_k16_pack_rhs(const uint16_t *d_X, int n, void *d_Xb, bgb_stream_t stream) {
+  clear_error();
+  BGB_REQUIRE(n >= 0 && d_Xb && (n == 0 || d_X), BGB_ERR_ARG, "bad RHS arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const long long nb16 = ceil_div(n, 16);
+  k16_pack_rhs_kernel<<<nb256(nb16 * 256), 256, 0, s>>>(d_X, n, nb16, static_cast<uint16_t *>(d_Xb));
+  BGB_HIP_TRY(hipGetLastError());
+  return BGB_OK;
+}
+
+int bgb_bmv_bin_full_full_k16(int T, const void *d_A, const void *d_Xb, float *d_Y, const int *d_rowptr,
+                              const int *d_colind, int nblockrows, int nblocks, int accumulate,
+                              bgb_stream_t stream) {
+  clear_error();
+  int st = k16_check(T, d_A, d_Xb, d_Y, d_rowptr, d_colind, nblockrows, nblocks);
+  if (st) return st;
+  BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
+  if (nblockrows == 0) return BGB_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nblocks == 0) {
+    if (!accumulate)
+      k16_fill_kernel<float><<<nb256((long long)nblockrows * T * kK), 256, 0, s>>>(d_Y, (long long)nblockrows * T * kK, 0.f);
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  }
+  const long long nchunks = ceil_div(nblocks, kChunk);
+  const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
+  void *ws = nullptr;
+  if ((st = stream_workspace(s, key_bytes + sizeof(float) * 32 * kK * (size_t)nchunks, &ws))) return st;
+  K16Args a;
+  a.g = K16Graph{d_rowptr, d_colind, d_A, static_cast<const uint2 *>(d_Xb), d_Y, nblockrows, nblocks, accumulate};
+  a.nchunks = (int)nchunks;
+  a.carry_key = static_cast<int *>(ws);
+  a.carry_val = reinterpret_cast<float *>(static_cast<char *>(ws) + key_bytes);
+  const unsigned blocks = (unsigned)ceil_div(nchunks, kWaves);
+  if (T == 16) k16_kernel<16><<<blocks, 64 * kWaves, 0, s>>>(a);
+  else k16_kernel<32><<<blocks, 64 * kWaves, 0, s>>>(a);
+  BGB_HIP_TRY(hipGetLastError());
+  k16_fixup_kernel<<<nb256(nchunks * 32 * kK), 256, 0, s>>>(a.carry_key, a.carry_val, nchunks, nullptr, nullptr,
+                                                           nullptr, a.g, T);
+  BGB_HIP_TRY(hipGetLastError());
+  return BGB_OK;
+}
+
+int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_rowptr, const int *d_colind,
+                    const void *d_val, const void *d_Xb, float *d_Y, int nworkgroups, bgb_stream_t stream) {
+  clear_error();
+  BGB_REQUIRE(ngraphs >= 1 && h_graphs && d_rowptr && d_Y, BGB_ERR_ARG, "bad batch arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<K16Graph> gs(ngraphs);
+  std::vector<int> td(ngraphs);
+  std::vector<long long> cs(ngraphs + 1, 0);
+  for (int i = 0; i < ngraphs; ++i) {
+    const bgb_k16_graph &h = h_graphs[i];
+    BGB_REQUIRE(h.T == 16 || h.T == 32, BGB_ERR_TILEDIM, "graph %d: tile dim %d not in {16, 32}", i, h.T);
+    BGB_REQUIRE(h.n >= 1 && h.nblocks >= 0 && h.val_byte_off % 16 == 0 && h.x_off % 256 == 0, BGB_ERR_ARG,
+                "graph %d: bad sizes or misaligned offsets", i);
+    const int nbr = (int)ceil_div(h.n, h.T);
+    gs[i] = K16Graph{d_rowptr + h.rowptr_off, d_colind + h.tile_off,
+                     static_cast<const char *>(d_val) + h.val_byte_off,
+                     reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(d_Xb) + h.x_off),
+                     d_Y + h.y_off, nbr, h.nblocks, 0};
+    td[i] = h.T;
+    // a graph without tiles still owns one chunk, which writes its zero rows
+    cs[i + 1] = cs[i] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
+    if (h.nblocks == 0)
+      k16_fill_kernel<float><<<nb256((long long)nbr * h.T * kK), 256, 0, s>>>(d_Y + h.y_off, (long long)nbr * h.T * kK, 0.f);
+  }
+  const long long total = cs[ngraphs];
+  DeviceBuffer dg, dt, dc, q, ck, cv, go;
+  BGB_HIP_TRY(dg.alloc(sizeof(K16Graph) * ngraphs, s));
+  BGB_HIP_TRY(dt.alloc(sizeof(int) * ngraphs, s));
+  BGB_HIP_TRY(dc.alloc(sizeof(long long) * (ngraphs + 1), s));
+  BGB_HIP_TRY(q.alloc(sizeof(unsigned long long), s));
+  BGB_HIP_TRY(ck.alloc(sizeof(int) * (total + 1), s));
+  BGB_HIP_TRY(cv.alloc(sizeof(float) * 32 * kK * (total + 1), s));
+  BGB_HIP_TRY(go.alloc(sizeof(int) * (total + 1), s));
+  BGB_HIP_TRY(hipMemcpyAsync(dg.ptr, gs.data(), sizeof(K16Graph) * ngraphs, hipMemcpyHostToDevice, s));
+  BGB_HIP_TRY(hipMemcpyAsync(dt.ptr, td.data(), sizeof(int) * ngraphs, hipMemcpyHostToDevice, s));
+  BGB_HIP_TRY(hipMemcpyAsync(dc.ptr, cs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
+  k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 1, 0ull);
+  BGB_HIP_TRY(hipGetLastError());
+  if (total > 0) {
+    K16BatchArgs a{dg.as<K16Graph>(), dt.as<int>(), dc.as<long long>(), ngraphs, total, q.as<unsigned long long>(),
+                   ck.as<int>(), cv.as<float>(), go.as<int>()};
+    int wgs = nworkgroups > 0 ? nworkgroups : 2048;
+    if ((long long)wgs * kWaves > total) wgs = (int)ceil_div(total, kWaves);
+    k16_persistent_kernel<<<wgs, 64 * kWaves, 0, s>>>(a);
+    BGB_HIP_TRY(hipGetLastError());
+    K16Graph none{};
+    k16_fixup_kernel<<<nb256(total * 32 * kK), 256, 0, s>>>(ck.as<int>(), cv.as<float>(), total, dg.as<K16Graph>(),
+                                                           go.as<int>(), dt.as<int>(), none, 0);
+    BGB_HIP_TRY(hipGetLastError());
+  }
+  BGB_HIP_TRY(hipStreamSynchronize(s));
+  return BGB_OK;
+}
+
+}  // extern "C"

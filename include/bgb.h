@@ -132,6 +132,44 @@ int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_
                                bgb_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
+ * K = 16 right-hand sides on MFMA (no reference counterpart; the reference's *
+ * bin_full_full takes one FP32 vector). Tile dims 16 and 32. X is FP16      *
+ * (raw uint16 bits), 0/1 tiles are exact in FP16, accumulation is FP32.     *
+ * ------------------------------------------------------------------------ */
+
+/* Permutes X (row-major FP16 [n][16]) into the MFMA operand order used by
+ * the K=16 kernels: Xb[(b*64 + l)*4 + j] = X[16b + 4(l/16) + j][l % 16],
+ * ceil(n/16)*256 values, zero beyond n. */
+int bgb_k16_pack_rhs(const uint16_t *d_X, int n, void *d_Xb, bgb_stream_t stream);
+
+/* Y[v][0..16) (+)= sum_u A[v][u] X[u][0..16), Y FP32 row-major
+ * [nblockrows*T][16]. Xb from bgb_k16_pack_rhs over nblockrows*T rows. */
+int bgb_bmv_bin_full_full_k16(int T, const void *d_A, const void *d_Xb, float *d_Y,
+                              const int *d_rowptr, const int *d_colind, int nblockrows,
+                              int nblocks, int accumulate, bgb_stream_t stream);
+
+/* One graph of a batched K=16 launch: offsets into the shared arrays. */
+typedef struct {
+  int T;                  /* 16 or 32 */
+  int n;                  /* vertices (nblockrows = ceil(n/T)) */
+  int nblocks;
+  int reserved;
+  long long rowptr_off;   /* elements into d_rowptr (graph-local values) */
+  long long tile_off;     /* elements into d_colind (graph-local values) */
+  long long val_byte_off; /* bytes into d_val, multiple of 16 */
+  long long x_off;        /* FP16 elements into d_Xb, multiple of 256 */
+  long long y_off;        /* floats into d_Y */
+} bgb_k16_graph;
+
+/* Y = A X for every graph of the batch in one persistent launch: waves take
+ * tile chunks of all graphs from one queue until it is drained, so graphs of
+ * very different sizes share the machine. nworkgroups <= 0 picks a default.
+ * Overwrites Y. Synchronises. */
+int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_rowptr,
+                    const int *d_colind, const void *d_val, const void *d_Xb, float *d_Y,
+                    int nworkgroups, bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
  * BFS on a symmetric B2SR matrix (no reference counterpart; the reference's *
  * building block is the masked BMV above). n = number of vertices,         *
  * nblockrows = ceil(n/T). Levels are int32 hop counts, -1 = unreached.     *
@@ -161,6 +199,24 @@ int bgb_bfs_plain(int T, const void *d_A, const int *d_rowptr, const int *d_coli
 int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
                  int nblockrows, int nblocks, int n, float damping, int iters, int mode,
                  float *d_pr, double *h_residuals, bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * Tile-dim autotuner (host). The reference fixes T at compile time         *
+ * (Makefile:12-46); its tile statistics are debug kernels                  *
+ * (printMatrixStats_T, estimatevec, utility.cu:808-975).                   *
+ * ------------------------------------------------------------------------ */
+
+/* Exact number of B2SR tiles (and of non-empty block rows) of a host CSR for
+ * T = 4, 8, 16, 32 (arrays of 4, in that order; h_nonempty4 may be NULL). */
+int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colind,
+                   long long *h_nblocks4, long long *h_nonempty4);
+
+/* Picks T for an op family (0: bin_bin_*, 1: bin_full_full, 2: K=16 MFMA)
+ * from the tile counts with a cost model fitted to MI355X runs of these
+ * kernels. allowed_mask bit t enables T = 4 << t. cost4_us (may be NULL)
+ * receives the predicted microseconds per T. */
+int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int allowed_mask,
+                         int *T_out, double *cost4_us);
 
 /* ------------------------------------------------------------------------ *
  * Host graph generators and CSR normalisation (no reference counterpart    *

@@ -130,3 +130,29 @@ def test_entry_points_validate_before_touching_the_gpu():
     nb = ctypes.c_int()
     assert L.bgb_b2sr_nnzb(4, -1, 8, 0, vp(16), vp(16), vp(16), ctypes.byref(nb), None) == 1
     assert b"tile dim" in L.bgb_last_error() or L.bgb_last_error() != b""
+
+
+def test_tile_stats_match_oracle_packing(orc):
+    rng = np.random.default_rng(8)
+    for n, m in ((1, 0), (37, 50), (1000, 8000)):
+        g = graph.from_coo(n, n, rng.integers(0, n, m), rng.integers(0, n, m))
+        nb, ne = graph.tile_stats(g)
+        for T in (4, 8, 16, 32):
+            brp, _, _ = orc.b2sr(T, n, n, g.rowptr, g.colind)
+            assert nb[T] == brp[-1]
+            assert ne[T] == int(np.count_nonzero(np.diff(brp)))
+
+
+def test_autotuner_prefers_small_tiles_on_sparse_random_graphs():
+    g = graph.erdos_renyi(20000, 8, seed=3)
+    T, cost = graph.autotune_tiledim(g, "bin_bin")
+    assert T in (4, 8) and cost[T] == min(cost.values())
+    T16, _ = graph.autotune_tiledim(g, "k16")
+    assert T16 in (16, 32)
+    # a dense block structure favours big tiles: 32x32 all-ones blocks on the diagonal
+    ii, jj = np.meshgrid(np.arange(32), np.arange(32))
+    src = np.concatenate([ii.ravel() + 32 * b for b in range(200)])
+    dst = np.concatenate([jj.ravel() + 32 * b for b in range(200)])
+    d = graph.from_coo(6400, 6400, src, dst, False, False)
+    Td, _ = graph.autotune_tiledim(d, "bin_bin")
+    assert Td == 32
