@@ -129,6 +129,96 @@ def rocsparse_baseline(torch, csr, x01, reps=20):
     return out
 
 
+def bfs_section(torch, B, graph, csr, T, nsrc=64, reps=3, plain=True):
+    """Batched 64-source BFS (hipGraph level loop) vs the per-source masked-BMV loop."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    A = B.B2SR.from_csr(csr, T, dev)
+    deg = csr.degrees()
+    rng = np.random.default_rng(64)
+    src = rng.choice(np.nonzero(deg)[0], nsrc, replace=False).astype(np.int32)
+    lev, nl = B.bfs_multi(A, src)  # warm-up (also the result)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        lev, nl = B.bfs_multi(A, src)
+    e.record()
+    torch.cuda.synchronize()
+    t_batch = s.elapsed_time(e) / 1e3 / reps
+    # Graph500 numerator: undirected edges in each source's component
+    dg = torch.from_numpy(deg.astype(np.int64)).to(dev)
+    edges = int(((lev >= 0).to(torch.int64) * dg).sum().item()) // 2
+    out = {"tile_dim": T, "sources": nsrc, "levels": nl, "edges_traversed": edges,
+           "batched_ms": round(t_batch * 1e3, 3), "batched_gteps": round(edges / t_batch / 1e9, 3)}
+    log(f"[bench] BFS batched T={T}: {t_batch * 1e3:.3f} ms, {nl} levels, {edges / t_batch / 1e9:.2f} GTEPS")
+    if plain:
+        plv = torch.empty((nsrc, csr.nrows), dtype=torch.int32, device=dev)
+        B.bfs_plain(A, int(src[0]), out=plv[0])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.record()
+        for i in range(nsrc):
+            B.bfs_plain(A, int(src[i]), out=plv[i])
+        e.record()
+        torch.cuda.synchronize()
+        t_plain = s.elapsed_time(e) / 1e3
+        same = bool(torch.equal(plv, lev))
+        out.update({"plain_ms": round(t_plain * 1e3, 3), "plain_gteps": round(edges / t_plain / 1e9, 3),
+                    "speedup": round(t_plain / t_batch, 2), "identical_levels": same})
+        log(f"[bench] BFS plain loop: {t_plain * 1e3:.1f} ms ({edges / t_plain / 1e9:.2f} GTEPS), "
+            f"speedup {t_plain / t_batch:.1f}x, identical={same}")
+    del A, lev
+    torch.cuda.empty_cache()
+    return out
+
+
+def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True):
+    """bin_full_full BMV + PageRank (three precision modes) on the side x side grid."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    csr = graph.grid2d(side, side)
+    if T is None:
+        T, _ = graph.autotune_tiledim(csr, "bin_full_full")
+    A = B.B2SR.from_csr(csr, T, dev)
+    x = graph.uniform(csr.nrows, 0.0, 1.0, 3)
+    xp = B.padded_vector(x, T, dev)
+    C = torch.empty(A.nblockrows * T, dtype=torch.float32, device=dev)
+    t, _ = time_bmv(torch, lambda: B.bmv_bin_full_full(A, xp, out=C), 20, 3)
+    nbr = A.nblockrows
+    nbytes = 4 * (nbr + 1) + 4 * A.nblocks + A.nblocks * T * {4: 1, 8: 1, 16: 2, 32: 4}[T] + 8 * nbr * T
+    out = {"tile_dim": T, "n": csr.nrows, "nnz": csr.nnz, "nblocks": A.nblocks,
+           "bmv_ms": round(t * 1e3, 4), "bmv_gteps": round(csr.nnz / t / 1e9, 2),
+           "bmv_alg_gbs": round(nbytes / t / 1e9, 1), "bmv_hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"[bench] grid {side}^2 bin_full_full T={T}: {t * 1e3:.4f} ms {csr.nnz / t / 1e9:.1f} GTEPS "
+        f"{nbytes / t / 1e9:.0f} GB/s")
+    ref = None
+    if with_cpu:
+        import oracle_lib
+        orc = oracle_lib.Oracle()
+        t0 = time.perf_counter()
+        ref, _ = orc.pagerank(csr.rowptr, csr.colind, 0.85, iters)
+        out["cpu_fp64_s"] = round(time.perf_counter() - t0, 2)
+    modes = {}
+    for mode in ("fp32", "bf16", "fp64sum"):
+        B.pagerank(A, 0.85, 2, mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr, res = B.pagerank(A, 0.85, iters, mode)
+        el = time.perf_counter() - t0
+        m = {"ms": round(el * 1e3, 2), "final_residual": float(res[-1])}
+        if ref is not None:
+            p = pr.cpu().numpy().astype(np.float64)
+            m["max_rel_err_vs_fp64"] = float(np.max(np.abs(p - ref) / ref))
+            m["l1_err_vs_fp64"] = float(np.abs(p - ref).sum())
+        modes[mode] = m
+        log(f"[bench] pagerank {mode}: {m}")
+    out["pagerank"] = modes
+    del A, xp, C
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +227,13 @@ def main():
     ap.add_argument("--config", default="rmat20", choices=sorted(CONFIGS))
     ap.add_argument("--tiles", default="4,8,16,32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--extras", default="", help="comma list: bfs, rmat22, pagerank")
+    ap.add_argument("--bfs-tile", type=int, default=0, help="tile dim for the BFS section (0: autotune)")
+    ap.add_argument("--run-length", type=int, default=0,
+                    help="tiles per lane group in the BMV kernels (8 or 16; 0 keeps the library default)")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling workload only: a 256 MiB copy (counter calibration) then --steps BMVs "
+                         "at the first of --tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -146,9 +243,15 @@ def main():
 
     import numpy as np
     import torch
+
+    import bgb
     from bgb import b2sr as B
     from bgb import graph
 
+    if args.run_length:
+        for T in (4, 8, 16, 32):
+            bgb.call("bgb_set_run_length", T, args.run_length)
+    run_lengths = {T: bgb.lib().bgb_get_run_length(T) for T in (4, 8, 16, 32)}
     cfg = CONFIGS[args.config]
     log(f"[bench] generating R-MAT scale {cfg['scale']} ef {cfg['edge_factor']}")
     t0 = time.perf_counter()
@@ -157,6 +260,23 @@ def main():
     log(f"[bench] n={csr.nrows} nnz={csr.nnz} gen {time.perf_counter() - t0:.1f}s")
     dev = torch.device("cuda:0")
     tiles = [int(t) for t in args.tiles.split(",")]
+
+    if args.profile:
+        T = tiles[0]
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB read + 256 MiB written
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        A = B.B2SR.from_csr(csr, T, dev)
+        xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+        C = torch.empty(A.nblockrows * T, dtype=torch.int32, device=dev)
+        t, _ = time_bmv(torch, lambda: B.bmv_bin_bin_full(A, xb, out=C), args.steps, 1)
+        nbytes = bmv_bytes(T, A.nblockrows, A.nblocks)
+        log(f"[profile] T={T} nblocks={A.nblocks} alg_bytes={nbytes} event_ms={t * 1e3:.4f} "
+            f"copy_bytes_each_way={4 * a.numel()}")
+        print(json.dumps({"profile": True, "tile_dim": T, "alg_bytes": nbytes, "event_ms": t * 1e3,
+                          "nblocks": A.nblocks, "nnz": csr.nnz}), flush=True)
+        return
 
     per_tile = {}
     results = {}
@@ -199,6 +319,36 @@ def main():
                "gpu_matches_cpu": bool(match)}
         log(f"[bench] cpu {cpu['value']:.3f} GTEPS on {cpu['cores']} threads, match={match}")
 
+    extras = {}
+    wanted = [w for w in args.extras.split(",") if w]
+    if "rmat22" in wanted or "bfs" in wanted:
+        log("[bench] generating R-MAT scale 22 ef 16")
+        t0 = time.perf_counter()
+        csr22 = graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1)
+        log(f"[bench] n={csr22.nrows} nnz={csr22.nnz} gen {time.perf_counter() - t0:.1f}s")
+        if "rmat22" in wanted:
+            x22 = graph.bernoulli(csr22.nrows, 0.05, 2)
+            pt22 = {}
+            for T in tiles:
+                A = B.B2SR.from_csr(csr22, T, dev)
+                xb = B.pack_vec(B.padded_vector(x22, T, dev), T)
+                C = torch.empty(A.nblockrows * T, dtype=torch.int32, device=dev)
+                t, _ = time_bmv(torch, lambda: B.bmv_bin_bin_full(A, xb, out=C), args.steps, args.warmup)
+                nbytes = bmv_bytes(T, A.nblockrows, A.nblocks)
+                pt22[str(T)] = {"ms": round(t * 1e3, 4), "gteps": round(csr22.nnz / t / 1e9, 2),
+                                "alg_gbs": round(nbytes / t / 1e9, 1),
+                                "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "nblocks": A.nblocks}
+                log(f"[bench] rmat22 T={T}: {pt22[str(T)]}")
+                del A, xb, C
+                torch.cuda.empty_cache()
+            extras["bin_bin_full_rmat22"] = {"nnz": csr22.nnz, "per_tile": pt22}
+        if "bfs" in wanted:
+            bt = args.bfs_tile or graph.autotune_tiledim(csr22, "bin_bin")[0]
+            extras["bfs_rmat22"] = bfs_section(torch, B, graph, csr22, bt)
+        del csr22
+    if "pagerank" in wanted:
+        extras["pagerank_grid4096"] = pagerank_section(torch, B, graph, with_cpu=not args.no_cpu)
+
     b = per_tile[best]
     line = {
         "metric": METRIC,
@@ -217,6 +367,7 @@ def main():
             "workload": (f"bin_bin_full BMV, R-MAT scale {cfg['scale']} ef {cfg['edge_factor']} "
                          "(a=.57 b=.19 c=.19) symmetrised, x Bernoulli(0.05), best tile dim"),
             "tile_dim": best, "n": csr.nrows, "nnz": csr.nnz, "nblocks": b["nblocks"],
+            "run_length": run_lengths[best],
         },
         "roofline": {"bound": "hbm", "achieved": round(b["alg_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4), "traffic": None},
@@ -226,6 +377,7 @@ def main():
         "per_tile": {str(k): {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                               for kk, vv in v.items()} for k, v in per_tile.items()},
     }
+    line.update(extras)
     print(json.dumps(line), flush=True)
 
 

@@ -383,26 +383,53 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   int h_ctl[4] = {0, 0, 0, 0};
   const int max_pairs = (1 << kPlanes) / 2 - 1;
   if (use_graph) {
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    BGB_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    level_pair();
-    BGB_HIP_TRY(hipStreamEndCapture(s, &graph));
-    BGB_HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    int pairs = 0;
+    // Capture is not allowed on the legacy default stream: capture and replay
+    // on a private non-blocking stream (default priority), ordered after the
+    // setup work queued on s.
+    BGB_HIP_TRY(hipStreamSynchronize(s));
+    hipStream_t cs = nullptr;
+    BGB_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     int st_code = BGB_OK;
-    while (pairs < max_pairs) {
-      // replay a few pairs between host checks; finished levels are no-ops
-      for (int r = 0; r < 4; ++r, ++pairs)
-        if (hipGraphLaunch(exec, s) != hipSuccess) st_code = BGB_ERR_HIP;
-      if (hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
+    const char *what = "";
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      st_code = BGB_ERR_HIP;
+      what = "hipStreamBeginCapture";
+    } else {
+      const hipStream_t saved = s;
+      s = cs;
+      level_pair();
+      s = saved;
+      if (hipStreamEndCapture(cs, &graph) != hipSuccess) {
         st_code = BGB_ERR_HIP;
-      if (st_code || h_ctl[2]) break;
+        what = "hipStreamEndCapture";
+      } else if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        st_code = BGB_ERR_HIP;
+        what = "hipGraphInstantiate";
+      }
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (st_code) return set_error(st_code, "BFS graph replay failed");
+    int pairs = 0;
+    while (st_code == BGB_OK && pairs < max_pairs) {
+      // replay a few pairs between host checks; levels after the last are no-ops
+      for (int r = 0; r < 4 && st_code == BGB_OK; ++r, ++pairs)
+        if (hipGraphLaunch(exec, cs) != hipSuccess) {
+          st_code = BGB_ERR_HIP;
+          what = "hipGraphLaunch";
+        }
+      if (st_code == BGB_OK &&
+          (hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+           hipStreamSynchronize(cs) != hipSuccess)) {
+        st_code = BGB_ERR_HIP;
+        what = "level flag read-back";
+      }
+      if (h_ctl[2]) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+    if (st_code) return set_error(st_code, "BFS level graph: %s failed", what);
   } else {
     for (int pairs = 0; pairs < max_pairs; ++pairs) {
       level_pair();

@@ -60,7 +60,11 @@ struct BmvArgs {
 };
 
 constexpr int kWavesPerBlock = 4;
-constexpr int kRun = 8;  // tiles per lane-group run (S)
+
+// Tiles per lane-group run (S), per tile dim (index log2(T)-2): 8 or 16.
+// Longer runs put more bytes in flight per wave, which matters for the small
+// tiles; set with bgb_set_run_length.
+int g_run_length[4] = {16, 16, 8, 8};
 
 template <int OP>
 struct AccOf {
@@ -126,22 +130,22 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   const bool active = pa < pb;
   const int *__restrict__ rowptr = a.rowptr;
 
-  // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
-  const int Ia = wave_find_row(rowptr, 0, a.nbr - 1, k0, lane);
-  const int Ib = wave_find_row(rowptr, Ia, a.nbr - 1, lmin(k0 + CH, a.nblocks) - 1, lane);
-  int I = active ? group_find_row(rowptr, Ia, Ib, pa) : 0;
-  const int I0 = I;
-
+  // Tile data first: it does not depend on the row search, so its latency
+  // overlaps the search below.
   uint32_t F[S][NW];
   int J[S];
   const int n = (int)(pb - pa);
   if (active) load_run<T, S>(a.A, a.colind, pa, n, q, F, J);
-
   uint32_t X[S];
   if constexpr (!is_full(OP)) {
 #pragma unroll
     for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
   }
+
+  // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
+  const int2 ab = wave_find_rows2(rowptr, a.nbr - 1, k0, lmin(k0 + CH, a.nblocks) - 1, lane);
+  int I = active ? group_find_row(rowptr, ab.x, ab.y, pa) : 0;
+  const int I0 = I;
 
   Acc acc[R], zero[R];
 #pragma unroll
@@ -287,9 +291,9 @@ __global__ void fill_kernel(V *p, long long n, V v) {
   if (i < n) p[i] = v;
 }
 
-template <int T, int OP>
-int launch_bmv(BmvArgs a, hipStream_t s) {
-  constexpr int S = kRun, CH = (64 / Tile<T>::L) * S;
+template <int T, int OP, int S>
+int launch_bmv_s(BmvArgs a, hipStream_t s) {
+  constexpr int CH = (64 / Tile<T>::L) * S;
   using Acc = typename AccOf<OP>::type;
   const long long nchunks = ceil_div(a.nblocks, CH);
   BGB_REQUIRE(nchunks < INT_MAX / 2, BGB_ERR_SIZE, "too many chunks");
@@ -307,6 +311,11 @@ int launch_bmv(BmvArgs a, hipStream_t s) {
   bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
+}
+
+template <int T, int OP>
+int launch_bmv(BmvArgs a, hipStream_t s) {
+  return g_run_length[ilog2_tiledim(T) - 2] == 16 ? launch_bmv_s<T, OP, 16>(a, s) : launch_bmv_s<T, OP, 8>(a, s);
 }
 
 template <int OP>
@@ -385,6 +394,16 @@ int stream_workspace(hipStream_t s, size_t bytes, void **out) {
 using namespace bgb;
 
 extern "C" {
+
+int bgb_set_run_length(int T, int S) {
+  clear_error();
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(S == 8 || S == 16, BGB_ERR_ARG, "run length must be 8 or 16 (got %d)", S);
+  g_run_length[ilog2_tiledim(T) - 2] = S;
+  return BGB_OK;
+}
+
+int bgb_get_run_length(int T) { return valid_tiledim(T) ? g_run_length[ilog2_tiledim(T) - 2] : -1; }
 
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C, const int *d_rowptr,
                         const int *d_colind, int nblockrows, int nblocks, int accumulate,

@@ -143,6 +143,31 @@ __device__ __forceinline__ int wave_find_row(const int *__restrict__ rowptr, int
   return lo;
 }
 
+// The rows holding tiles p0 <= p1, found together: lanes 0-31 search for p0,
+// lanes 32-63 for p1, each half probing 32 entries per step. Must be called
+// by all 64 lanes; returns the pair in every lane.
+__device__ __forceinline__ int2 wave_find_rows2(const int *__restrict__ rowptr, int hi0, long long p0, long long p1,
+                                                int lane) {
+  const int half = lane >> 5, hl = lane & 31;
+  const long long p = half ? p1 : p0;
+  int lo = 0, hi = hi0;
+  for (;;) {
+    const bool open = hi > lo;
+    if (!__any(open)) break;
+    const long long span = hi - lo;
+    const int idx = lo + (int)((span * (hl + 1)) >> 5);
+    const unsigned long long m = __ballot(rowptr[idx] <= p);
+    const int cnt = __popc(half ? (unsigned)(m >> 32) : (unsigned)m);
+    if (open) {
+      const int nlo = cnt > 0 ? lo + (int)((span * cnt) >> 5) : lo;
+      const int nhi = cnt < 32 ? lo + (int)((span * (cnt + 1)) >> 5) - 1 : hi;
+      lo = nlo;
+      hi = nhi;
+    }
+  }
+  return make_int2(__shfl(lo, 0), __shfl(lo, 32));
+}
+
 // Row of tile pa for one lane group, searched inside the wave's row range [lo, hi].
 __device__ __forceinline__ int group_find_row(const int *__restrict__ rowptr, int lo, int hi, long long pa) {
   while (lo < hi) {

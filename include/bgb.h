@@ -86,6 +86,12 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
  * use separate per-stream workspaces.                                       *
  * ------------------------------------------------------------------------ */
 
+/* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
+ * tuning knob of the BMV kernels (no reference counterpart). Defaults: 16 for
+ * T = 4, 8 and 8 for T = 16, 32. */
+int bgb_set_run_length(int T, int S);
+int bgb_get_run_length(int T);
+
 /* C[I] (|)= bits(T-1-r) = OR_c A[I*T+r][c] & x[c].  bmv{T}_bin_bin_bin
  * (bsrbmv.cu:277,437,508,579). B: packed bit vector, C: W_T[NBR]. */
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C,

@@ -15,6 +15,8 @@ TS = (4, 8, 16, 32)
 # PageRank tolerances vs the FP64 oracle: FP32 storage gives ~1e-7 per step,
 # damped by 1/(1-d); bf16 x carries a 2^-9 relative rounding per element.
 PR_RTOL = {"fp32": 2e-5, "fp64sum": 2e-5, "bf16": 2e-2}
+# total rank mass |sum(pr) - 1|: bf16 rounding of x biases the mass by up to ~2^-9
+PR_MASS = {"fp32": 1e-4, "fp64sum": 1e-4, "bf16": 5e-3}
 
 
 def log(msg):
@@ -104,7 +106,7 @@ def test_pagerank_grid_modes(orc, dev, T):
         pr, res = B.pagerank(A, 0.85, 50, mode)
         err = pr_errors(pr.cpu().numpy(), ref)
         assert err < PR_RTOL[mode], (mode, err)
-        assert abs(float(pr.double().sum()) - 1.0) < 1e-4
+        assert abs(float(pr.double().sum()) - 1.0) < PR_MASS[mode]
         # residual history follows the FP64 run (same decay)
         np.testing.assert_allclose(res[:10], rres[:10], rtol=0.05 if mode == "bf16" else 1e-3)
 
