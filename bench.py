@@ -67,6 +67,25 @@ def copy_ceiling_gbs(torch, nbytes=1 << 30, reps=10):
     return 2 * nbytes / t / 1e9
 
 
+def read_ceiling_gbs(torch, nbytes=1 << 30, reps=10):
+    """Read-only HBM stream (16-byte nontemporal loads) on this box, GB/s."""
+    import ctypes
+    path = os.path.join(REPO, "bit-graphblas_amd", "lib", "libbgb_baseline.so")
+    if not os.path.exists(path):
+        return None
+    f = ctypes.CDLL(path).bgbx_stream_read
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    best = 0.0
+    for blocks in (2048, 4096, 8192):
+        ms = ctypes.c_double()
+        if f(a.data_ptr(), nbytes, blocks, reps, ctypes.byref(ms), torch.cuda.current_stream().cuda_stream) == 0:
+            best = max(best, nbytes / (ms.value / 1e3) / 1e9)
+    del a
+    return best
+
+
 def time_bmv(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -302,7 +321,8 @@ def main():
 
     best = max(per_tile, key=lambda k: per_tile[k]["gteps"])
     ceiling = copy_ceiling_gbs(torch)
-    log(f"[bench] copy ceiling {ceiling:.0f} GB/s")
+    rceil = read_ceiling_gbs(torch)
+    log(f"[bench] copy ceiling {ceiling:.0f} GB/s, read ceiling {rceil:.0f} GB/s")
 
     rs = rocsparse_baseline(torch, csr, x01)
     log(f"[bench] rocSPARSE csrmv: {rs}")
@@ -373,6 +393,7 @@ def main():
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4), "traffic": None},
         "cpu_baseline": cpu,
         "copy_ceiling_gbs": round(ceiling, 1),
+        "read_ceiling_gbs": round(rceil, 1) if rceil else None,
         "rocsparse_csrmv": rs,
         "per_tile": {str(k): {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                               for kk, vv in v.items()} for k, v in per_tile.items()},
