@@ -113,10 +113,6 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   const long long pb = lmin2(pa + S, b.nblocks);
   const bool active = pa < pb;
   const int *__restrict__ rowptr = b.rowptr;
-  const int Ia = wave_find_row(rowptr, 0, b.nbr - 1, k0, lane);
-  const int Ib = wave_find_row(rowptr, Ia, b.nbr - 1, lmin2(k0 + CH, b.nblocks) - 1, lane);
-  int I = active ? group_find_row(rowptr, Ia, Ib, pa) : 0;
-  const int I0 = I;
   uint32_t F[S][NW];
   int J[S];
   const int n = (int)(pb - pa);
@@ -124,6 +120,9 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   uint32_t X[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(b.fnz)[J[s]] : 0u;
+  const int2 ab = wave_find_rows2(rowptr, b.nbr - 1, k0, lmin2(k0 + CH, b.nblocks) - 1, lane);
+  int I = groups_find_row(rowptr, ab.x, ab.y, pa, active, lane);
+  const int I0 = I;
 
   bool any = false;
   uint64_t acc[R], zero[R], vr[R], hv[R], hvr[R];

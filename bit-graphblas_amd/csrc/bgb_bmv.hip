@@ -64,7 +64,7 @@ constexpr int kWavesPerBlock = 4;
 // Tiles per lane-group run (S), per tile dim (index log2(T)-2): 8 or 16.
 // Longer runs put more bytes in flight per wave, which matters for the small
 // tiles; set with bgb_set_run_length.
-int g_run_length[4] = {16, 16, 8, 8};
+int g_run_length[4] = {8, 8, 8, 8};
 
 template <int OP>
 struct AccOf {
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 
   // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
   const int2 ab = wave_find_rows2(rowptr, a.nbr - 1, k0, lmin(k0 + CH, a.nblocks) - 1, lane);
-  int I = active ? group_find_row(rowptr, ab.x, ab.y, pa) : 0;
+  int I = groups_find_row(rowptr, ab.x, ab.y, pa, active, lane);
   const int I0 = I;
 
   Acc acc[R], zero[R];
@@ -240,10 +240,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
     for (int i = 0; i < R; ++i) hv[i] += cin[i];
     flush_row<T, OP>(a, held, q, hv);
   }
-  // The last active group publishes the chunk's open row.
+  // The last active group publishes the chunk's open row, unless its partial
+  // is all zero (then there is nothing to add).
   const int last = (int)lmin(G - 1, (a.nblocks - 1 - k0) / S);
   if (g == last) {
-    const bool open = key < a.nbr;
+    bool nz = false;
+#pragma unroll
+    for (int i = 0; i < R; ++i) nz |= sc[i] != (Acc)0;
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
+    const bool open = key < a.nbr && nz;
     if (q == 0) a.carry_key[chunk] = open ? key : -1;
     if (open) {
       Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
@@ -253,18 +259,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   }
 }
 
-// Adds the chunk carries of rows that cross chunk boundaries.
+// Adds the chunk carries of rows that cross chunk boundaries: one thread per
+// chunk; chunks without a (non-zero) carry exit at once.
 template <int T, int OP>
 __global__ void bmv_fixup_kernel(BmvArgs a) {
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nchunks) return;
+  const int key = a.carry_key[c];
+  if (key < 0) return;
+  const Acc *cv = static_cast<const Acc *>(a.carry_val) + c * T;
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
-    if (tid >= a.nchunks) return;
-    const int key = a.carry_key[tid];
-    if (key < 0) return;
-    const Acc *cv = static_cast<const Acc *>(a.carry_val) + tid * T;
     uint32_t bits = 0;
+#pragma unroll
     for (int r = 0; r < T; ++r) bits |= (cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
     if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[key];
     if (bits == 0) return;
@@ -273,15 +281,13 @@ __global__ void bmv_fixup_kernel(BmvArgs a) {
     const int shift = (int)(addr & 3) * 8;
     atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << shift);
   } else {
-    if (tid >= (long long)a.nchunks * T) return;
-    const long long c = tid / T;
-    const int r = (int)(tid % T);
-    const int key = a.carry_key[c];
-    if (key < 0) return;
-    const Acc v = static_cast<const Acc *>(a.carry_val)[tid];
-    if (v == (Acc)0) return;
-    if constexpr (OP == OP_COUNT_I32) atomicAdd(static_cast<int *>(a.C) + (long long)key * T + r, (int)v);
-    else unsafeAtomicAdd(static_cast<float *>(a.C) + (long long)key * T + r, (float)v);
+#pragma unroll
+    for (int r = 0; r < T; ++r) {
+      const Acc v = cv[r];
+      if (v == (Acc)0) continue;
+      if constexpr (OP == OP_COUNT_I32) atomicAdd(static_cast<int *>(a.C) + (long long)key * T + r, (int)v);
+      else unsafeAtomicAdd(static_cast<float *>(a.C) + (long long)key * T + r, (float)v);
+    }
   }
 }
 
@@ -307,8 +313,7 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
   bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
-  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : nchunks * T;
-  bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
+  bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(nchunks, 256), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }

@@ -36,30 +36,33 @@ struct KeyLayout {
   int end_bit() const { return bitsI + bitsJ + 2 * lg; }
 };
 
-__global__ void validate_csr_kernel(const int *__restrict__ rowptr, const int *__restrict__ colind,
-                                    int nrows, int ncols, int nnz, int *__restrict__ bad) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// CSR checks, parallel over rows and over nonzeros (a per-row loop would
+// serialise on hub rows): rowptr bounds/monotonicity per row, row-start marks,
+// then per nonzero the column range and strict increase inside its row.
+__global__ void validate_rows_kernel(const int *__restrict__ rowptr, int nrows, int nnz,
+                                     uint8_t *__restrict__ row_start, int *__restrict__ bad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nrows) return;
-  int s = rowptr[i];
+  const int s = rowptr[i];
   if (i == 0 && s != 0) atomicOr(bad, 1);
   if (i == nrows) {
     if (s != nnz) atomicOr(bad, 2);
     return;
   }
-  int e = rowptr[i + 1];
+  const int e = rowptr[i + 1];
   if (e < s || e > nnz || s < 0) {
     atomicOr(bad, 4);
     return;
   }
-  int prev = -1;
-  for (int k = s; k < e; ++k) {
-    int j = colind[k];
-    if (j <= prev || j >= ncols) {
-      atomicOr(bad, 8);
-      return;
-    }
-    prev = j;
-  }
+  if (s < e) row_start[s] = 1;
+}
+
+__global__ void validate_cols_kernel(const int *__restrict__ colind, long long nnz, int ncols,
+                                     const uint8_t *__restrict__ row_start, int *__restrict__ bad) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int j = colind[k];
+  if (j < 0 || j >= ncols || (k > 0 && !row_start[k] && colind[k - 1] >= j)) atomicOr(bad, 8);
 }
 
 // One wave per CSR row; lanes stride over the row's nonzeros.
@@ -182,12 +185,18 @@ int check_common(int T, int nrows, int ncols, int nnz, const int *rowptr, const 
 
 // Checks the CSR structure on the device (synchronises the stream).
 int validate_csr(const int *rowptr, const int *colind, int nrows, int ncols, int nnz, hipStream_t s) {
-  DeviceBuffer bad;
+  DeviceBuffer bad, starts;
   BGB_HIP_TRY(bad.alloc(sizeof(int), s));
+  BGB_HIP_TRY(starts.alloc((size_t)nnz + 1, s));
   BGB_HIP_TRY(hipMemsetAsync(bad.ptr, 0, sizeof(int), s));
-  validate_csr_kernel<<<grid_for((long long)nrows + 1), kBlock, 0, s>>>(rowptr, colind, nrows, ncols,
-                                                                       nnz, bad.as<int>());
+  BGB_HIP_TRY(hipMemsetAsync(starts.ptr, 0, (size_t)nnz + 1, s));
+  validate_rows_kernel<<<grid_for((long long)nrows + 1), kBlock, 0, s>>>(rowptr, nrows, nnz, starts.as<uint8_t>(),
+                                                                        bad.as<int>());
   BGB_HIP_TRY(hipGetLastError());
+  if (nnz > 0) {
+    validate_cols_kernel<<<grid_for(nnz), kBlock, 0, s>>>(colind, nnz, ncols, starts.as<uint8_t>(), bad.as<int>());
+    BGB_HIP_TRY(hipGetLastError());
+  }
   int h_bad = 0;
   BGB_HIP_TRY(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   BGB_HIP_TRY(hipStreamSynchronize(s));

@@ -178,6 +178,32 @@ __device__ __forceinline__ int group_find_row(const int *__restrict__ rowptr, in
   return lo;
 }
 
+// Row of each lane group's first tile pa (rowptr[I] <= pa < rowptr[I+1]),
+// given the rows [Ia, Ib] holding the wave's first and last tiles. When the
+// wave spans fewer than 64 rows (the common case) one coalesced load brings
+// the row ends rowptr[Ia+1 .. Ib] into lanes and each lane binary-searches
+// that register window with shuffles; otherwise it binary-searches memory.
+// Must be called by all 64 lanes.
+__device__ __forceinline__ int groups_find_row(const int *__restrict__ rowptr, int Ia, int Ib, long long pa,
+                                               bool active, int lane) {
+  const int m = Ib - Ia;  // wave-uniform
+  if (m < 64) {
+    const int w = lane < m ? rowptr[Ia + 1 + lane] : 0x7fffffff;
+    int lo = 0, hi = m;  // number of row ends <= pa, in [lo, hi]
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+      const int mid = (lo + hi + 1) >> 1;
+      const int wm = __shfl(w, mid > 0 ? mid - 1 : 0);
+      if (mid > lo) {
+        if (wm <= pa) lo = mid;
+        else hi = mid - 1;
+      }
+    }
+    return Ia + lo;
+  }
+  return active ? group_find_row(rowptr, Ia, Ib, pa) : Ia;
+}
+
 // Workgroup index remapped so that consecutive index ranges stay on one XCD
 // (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
 __device__ __forceinline__ int xcd_remap(int b, int n) {

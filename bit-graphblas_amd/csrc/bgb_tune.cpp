@@ -28,10 +28,13 @@ struct Model {
   double bw_gbs;
 };
 // op 0: bin_bin_full / bin_bin_bin, op 1: bin_full_full (FP32), op 2: K=16 MFMA
+// Fitted to MI355X runs of the chunked kernels on R-MAT scale 20 / 22 and the
+// 4096^2 grid (whole-chip time per tile at full occupancy; the small tiles are
+// issue/latency bound, T = 32 is bandwidth bound at ~4.2 TB/s).
 const Model kModels[3] = {
-    {{0.0030, 0.0045, 0.0090, 0.0250}, {0.0040, 0.0060, 0.0120, 0.0240}, 5200.0},
-    {{0.0060, 0.0110, 0.0250, 0.0800}, {0.0050, 0.0080, 0.0150, 0.0300}, 5000.0},
-    {{1e9, 1e9, 0.0600, 0.2000}, {1e9, 1e9, 0.0300, 0.0600}, 4500.0},
+    {{0.0063, 0.0063, 0.0118, 0.0200}, {0.0020, 0.0030, 0.0050, 0.0100}, 4200.0},
+    {{0.0250, 0.0300, 0.0600, 0.1500}, {0.0030, 0.0050, 0.0100, 0.0200}, 4000.0},
+    {{1e9, 1e9, 0.0600, 0.2000}, {1e9, 1e9, 0.0300, 0.0600}, 4000.0},
 };
 
 double op_bytes(int op, int T, long long nrows, long long nblocks) {

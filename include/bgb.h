@@ -87,8 +87,7 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
  * ------------------------------------------------------------------------ */
 
 /* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
- * tuning knob of the BMV kernels (no reference counterpart). Defaults: 16 for
- * T = 4, 8 and 8 for T = 16, 32. */
+ * tuning knob of the BMV kernels (no reference counterpart). Default 8. */
 int bgb_set_run_length(int T, int S);
 int bgb_get_run_length(int T);
 
