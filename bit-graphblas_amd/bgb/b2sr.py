@@ -100,8 +100,18 @@ def _out(out, n, dtype, device, accumulate):
     return out
 
 
+def _bits_out(out, A):
+    """Bit-output words; fresh buffers are padded to whole dwords (pieces of rows
+    that cross chunks are OR-ed into the aligned dword that holds their word)."""
+    if out is not None:
+        return _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, True)
+    wb = lib().bgb_word_bytes(A.T)
+    words = -(-(A.nblockrows * wb) // 4) * 4 // wb
+    return torch.zeros(max(words, 1), dtype=WORD_TORCH[A.T], device=A.device)[: A.nblockrows]
+
+
 def bmv_bin_bin_bin(A, xb, out=None, accumulate=False):
-    C = _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, accumulate)
+    C = _bits_out(out, A)
     call("bgb_bmv_bin_bin_bin", A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr), _ptr(A.colind),
          A.nblockrows, A.nblocks, int(accumulate), _stream())
     return C
@@ -124,7 +134,7 @@ def bmv_bin_full_full(A, x, out=None, accumulate=False):
 
 
 def bmv_bin_bin_bin_masked(A, xb, mask, out=None):
-    C = _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, False)
+    C = _bits_out(out, A)
     call("bgb_bmv_bin_bin_bin_masked", A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr),
          _ptr(A.colind), A.nblockrows, A.nblocks, _ptr(mask), _stream())
     return C

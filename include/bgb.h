@@ -82,8 +82,10 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
  * rowptr[nblockrows] (passed so that no device read is needed).             *
  * accumulate = 1 reproduces the reference's in-place accumulation           *
  * (|= for bits, += for counts and sums); 0 overwrites C.                    *
- * Calls on one stream are ordered; calls that overlap on different streams  *
- * use separate per-stream workspaces.                                       *
+ * Outputs are 4-byte aligned; bit outputs (W_T[NBR]) are rounded up to a   *
+ * whole number of dwords: pieces of rows that cross the kernels' tile      *
+ * chunks are OR-ed / added atomically into them (C is zero-filled first    *
+ * unless accumulating). Needs no workspace.                                *
  * ------------------------------------------------------------------------ */
 
 /* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
