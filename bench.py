@@ -197,20 +197,26 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
     import numpy as np
     dev = torch.device("cuda:0")
     csr = graph.grid2d(side, side)
-    if T is None:
-        T, _ = graph.autotune_tiledim(csr, "bin_full_full")
-    A = B.B2SR.from_csr(csr, T, dev)
+    tuned, _ = graph.autotune_tiledim(csr, "bin_full_full")
     x = graph.uniform(csr.nrows, 0.0, 1.0, 3)
-    xp = B.padded_vector(x, T, dev)
-    C = torch.empty(A.nblockrows * T, dtype=torch.float32, device=dev)
-    t, _ = time_bmv(torch, lambda: B.bmv_bin_full_full(A, xp, out=C), 20, 3)
-    nbr = A.nblockrows
-    nbytes = 4 * (nbr + 1) + 4 * A.nblocks + A.nblocks * T * {4: 1, 8: 1, 16: 2, 32: 4}[T] + 8 * nbr * T
-    out = {"tile_dim": T, "n": csr.nrows, "nnz": csr.nnz, "nblocks": A.nblocks,
-           "bmv_ms": round(t * 1e3, 4), "bmv_gteps": round(csr.nnz / t / 1e9, 2),
-           "bmv_alg_gbs": round(nbytes / t / 1e9, 1), "bmv_hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
-    log(f"[bench] grid {side}^2 bin_full_full T={T}: {t * 1e3:.4f} ms {csr.nnz / t / 1e9:.1f} GTEPS "
-        f"{nbytes / t / 1e9:.0f} GB/s")
+    per_tile = {}
+    for Tt in (4, 8, 16, 32):
+        A = B.B2SR.from_csr(csr, Tt, dev)
+        xp = B.padded_vector(x, Tt, dev)
+        C = torch.empty(A.nblockrows * Tt, dtype=torch.float32, device=dev)
+        t, _ = time_bmv(torch, lambda: B.bmv_bin_full_full(A, xp, out=C), 20, 3)
+        nbr = A.nblockrows
+        nbytes = 4 * (nbr + 1) + 4 * A.nblocks + A.nblocks * Tt * {4: 1, 8: 1, 16: 2, 32: 4}[Tt] + 8 * nbr * Tt
+        per_tile[str(Tt)] = {"ms": round(t * 1e3, 4), "gteps": round(csr.nnz / t / 1e9, 2), "nblocks": A.nblocks,
+                             "alg_gbs": round(nbytes / t / 1e9, 1),
+                             "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] grid {side}^2 bin_full_full T={Tt}: {per_tile[str(Tt)]}")
+        del A, xp, C
+        torch.cuda.empty_cache()
+    T = T or tuned
+    A = B.B2SR.from_csr(csr, T, dev)
+    out = {"n": csr.nrows, "nnz": csr.nnz, "autotuned_tile_dim": tuned, "pagerank_tile_dim": T,
+           "bin_full_full_per_tile": per_tile}
     ref = None
     if with_cpu:
         import oracle_lib
@@ -233,7 +239,7 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
         modes[mode] = m
         log(f"[bench] pagerank {mode}: {m}")
     out["pagerank"] = modes
-    del A, xp, C
+    del A
     torch.cuda.empty_cache()
     return out
 
@@ -363,7 +369,7 @@ def main():
                 torch.cuda.empty_cache()
             extras["bin_bin_full_rmat22"] = {"nnz": csr22.nnz, "per_tile": pt22}
         if "bfs" in wanted:
-            bt = args.bfs_tile or graph.autotune_tiledim(csr22, "bin_bin")[0]
+            bt = args.bfs_tile or graph.autotune_tiledim(csr22, "bfs")[0]
             extras["bfs_rmat22"] = bfs_section(torch, B, graph, csr22, bt)
         del csr22
     if "pagerank" in wanted:

@@ -89,7 +89,7 @@ def tile_stats(csr):
     return {T: int(nb[i]) for i, T in enumerate((4, 8, 16, 32))}, {T: int(ne[i]) for i, T in enumerate((4, 8, 16, 32))}
 
 
-OP_FAMILY = {"bin_bin": 0, "bin_full_full": 1, "k16": 2}
+OP_FAMILY = {"bin_bin": 0, "bin_full_full": 1, "k16": 2, "bfs": 3}
 
 
 def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):

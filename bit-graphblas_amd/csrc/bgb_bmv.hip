@@ -15,13 +15,11 @@
 //   * the first row a group flushes may have started in an earlier group: a
 //     wave-level segmented scan over the groups' open rows supplies that
 //     carry, so rows that start and end inside one wave are written once;
-//   * rows that cross a chunk boundary are the only ones several waves add
-//     to: their pieces (the group's first row when it began in an earlier
-//     chunk, and the row still open at the chunk's end) are merged with
-//     atomics onto an output that a fill kernel zeroed first (integer add and
-//     32-bit OR are exact; FP32 uses float atomics as the reference does,
-//     bsrbmv.cu:895,971,1036); every other row is written once with plain
-//     stores. No carry buffer and no second pass.
+//   * a row still open at the end of the chunk leaves a carry record; a
+//     fix-up pass adds those records to C (integer atomics are exact, bit
+//     results use atomic OR, FP32 uses float atomics as the reference does);
+//     an all-zero record is not published. (Merging those rows with atomics
+//     inside the main pass instead was measured slower on MI355X.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,6 +57,8 @@ struct BmvArgs {
   int nblocks;
   int accumulate;
   int nchunks;
+  int *carry_key;
+  void *carry_val;
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -82,12 +82,9 @@ struct AccOf<OP_FULL_BF16> {
 };
 constexpr bool is_full(int op) { return op == OP_FULL_F32 || op == OP_FULL_BF16; }
 
-// Writes row I of the result from the lane group's partials. `merge` marks a
-// piece of a row that crosses chunks: it is added atomically (C was zeroed
-// unless accumulating), zero pieces are skipped.
 template <int T, int OP>
 __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
-                                          const typename AccOf<OP>::type (&v)[Tile<T>::R], bool merge) {
+                                          const typename AccOf<OP>::type (&v)[Tile<T>::R]) {
   constexpr int R = Tile<T>::R, L = Tile<T>::L;
   using W = typename Tile<T>::W;
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
@@ -99,36 +96,21 @@ __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
     if (q == 0) {
       W *C = static_cast<W *>(a.C);
       if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[I];
-      if (merge) {
-        if (bits) {
-          // OR into the aligned dword holding word I (W is 1, 2 or 4 bytes)
-          const uintptr_t addr = reinterpret_cast<uintptr_t>(C + I);
-          atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << ((int)(addr & 3) * 8));
-        }
-      } else {
-        if constexpr (OP != OP_MASKED)
-          if (a.accumulate) bits |= (uint32_t)C[I];
-        C[I] = (W)bits;
-      }
+      else if (a.accumulate) bits |= (uint32_t)C[I];
+      C[I] = (W)bits;
     }
   } else {
     using V = typename std::conditional<OP == OP_COUNT_I32, int, float>::type;
-    V *C = static_cast<V *>(a.C) + (long long)I * T + q * R;
-    if (merge) {
+    V out[R];
 #pragma unroll
-      for (int i = 0; i < R; ++i) {
-        if (v[i] == (typename AccOf<OP>::type)0) continue;
-        if constexpr (OP == OP_COUNT_I32) atomicAdd(C + i, (int)v[i]);
-        else unsafeAtomicAdd(C + i, (float)v[i]);
-      }
-      return;
-    }
+    for (int i = 0; i < R; ++i) out[i] = (V)v[i];
+    V *C = static_cast<V *>(a.C) + (long long)I * T + q * R;
     static_assert(R % 4 == 0, "rows per lane must fill 16-byte stores");
 #pragma unroll
     for (int j = 0; j < R / 4; ++j) {
       typedef V V4 __attribute__((ext_vector_type(4)));
       V4 *p = reinterpret_cast<V4 *>(C) + j;
-      V4 w = {(V)v[4 * j], (V)v[4 * j + 1], (V)v[4 * j + 2], (V)v[4 * j + 3]};
+      V4 w = {out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]};
       if (a.accumulate) w += *p;
       *p = w;
     }
@@ -161,6 +143,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 #pragma unroll
     for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
   }
+
   // FP ops with T <= 8: the whole x segment of each tile (T values) is loaded
   // up front with the tile, so the sums need no dependent gathers.
   constexpr bool kSeg = is_full(OP) && T <= 8;
@@ -208,7 +191,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 
   // Leading empty block rows end at position 0: the group holding tile 0 owns them.
   if (active && pa == 0)
-    for (int r = 0; r < I0; ++r) flush_row<T, OP>(a, r, q, zero, false);
+    for (int r = 0; r < I0; ++r) flush_row<T, OP>(a, r, q, zero);
 
   int held = -1;
   Acc hv[R];
@@ -260,14 +243,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 #pragma unroll
           for (int i = 0; i < R; ++i) hv[i] = acc[i];
         } else {
-          flush_row<T, OP>(a, I, q, acc, false);
+          flush_row<T, OP>(a, I, q, acc);
         }
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
         ++I;
         e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
         while (e == pos) {
-          flush_row<T, OP>(a, I, q, zero, false);
+          flush_row<T, OP>(a, I, q, zero);
           ++I;
           e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
         }
@@ -298,20 +281,71 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
     const Acc up = __shfl_up(sc[i], L);
     cin[i] = (g > 0 && kin == I0) ? up : (Acc)0;
   }
-  // The first row a group flushes crossed into this chunk from an earlier one
-  // iff it is the wave's first row and began before the chunk.
-  const bool first_split = ab.x == I0 && rowptr[I0] < k0;
   if (held >= 0) {
 #pragma unroll
     for (int i = 0; i < R; ++i) hv[i] += cin[i];
-    flush_row<T, OP>(a, held, q, hv, first_split);
+    flush_row<T, OP>(a, held, q, hv);
   }
-  // The row still open at the chunk's end continues in the next chunk: the
-  // last active group adds this chunk's piece of it.
+  // The last active group publishes the chunk's open row, unless its partial
+  // is all zero (then there is nothing to add).
   const int last = (int)lmin(G - 1, (a.nblocks - 1 - k0) / S);
-  if (g == last && key < a.nbr) flush_row<T, OP>(a, key, q, sc, true);
+  if (g == last) {
+    bool nz = false;
+#pragma unroll
+    for (int i = 0; i < R; ++i) nz |= sc[i] != (Acc)0;
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
+    const bool open = key < a.nbr && nz;
+    if (q == 0) a.carry_key[chunk] = open ? key : -1;
+    if (open) {
+      Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
+#pragma unroll
+      for (int i = 0; i < R; ++i) cv[i] = sc[i];
+    }
+  }
 }
 
+// Adds the chunk carries of rows that cross chunk boundaries after the main
+// pass. Chunks with no (non-zero) carry published key -1. Count / FP carries:
+// one thread per 4 consecutive values (one 16-byte load, independent
+// atomics); bit carries: one thread per chunk (one 32-bit OR).
+template <int T, int OP>
+__global__ void bmv_fixup_kernel(BmvArgs a) {
+  using Acc = typename AccOf<OP>::type;
+  using W = typename Tile<T>::W;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+    if (tid >= a.nchunks) return;
+    const int key = a.carry_key[tid];
+    if (key < 0) return;
+    const Acc *cv = static_cast<const Acc *>(a.carry_val) + tid * T;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int r = 0; r < T; ++r) bits |= (cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
+    if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[key];
+    if (bits == 0) return;
+    // OR into the aligned dword holding word `key` (W is 1, 2 or 4 bytes).
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(a.C) + key);
+    const int shift = (int)(addr & 3) * 8;
+    atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << shift);
+  } else {
+    constexpr int Q = T / 4;  // threads per chunk
+    if (tid >= (long long)a.nchunks * Q) return;
+    const long long c = tid / Q;
+    const int r0 = (int)(tid % Q) * 4;
+    const int key = a.carry_key[c];
+    if (key < 0) return;
+    typedef Acc Acc4 __attribute__((ext_vector_type(4)));
+    const Acc4 v = *reinterpret_cast<const Acc4 *>(static_cast<const Acc *>(a.carry_val) + c * T + r0);
+    const long long base = (long long)key * T + r0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (v[i] == (Acc)0) continue;
+      if constexpr (OP == OP_COUNT_I32) atomicAdd(static_cast<int *>(a.C) + base + i, (int)v[i]);
+      else unsafeAtomicAdd(static_cast<float *>(a.C) + base + i, (float)v[i]);
+    }
+  }
+}
 
 template <class V>
 __global__ void fill_kernel(V *p, long long n, V v) {
@@ -322,20 +356,21 @@ __global__ void fill_kernel(V *p, long long n, V v) {
 template <int T, int OP, int S>
 int launch_bmv_s(BmvArgs a, hipStream_t s) {
   constexpr int CH = (64 / Tile<T>::L) * S;
+  using Acc = typename AccOf<OP>::type;
   const long long nchunks = ceil_div(a.nblocks, CH);
   BGB_REQUIRE(nchunks < INT_MAX / 2, BGB_ERR_SIZE, "too many chunks");
   a.nchunks = (int)nchunks;
-  if (!a.accumulate || OP == OP_MASKED) {  // rows merged by atomics start from zero
-    const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)a.nbr * word_bytes(T)
-                                                           : (long long)a.nbr * T * 4;
-    fill_kernel<uint32_t><<<(unsigned)ceil_div(ceil_div(n, 4), 256), 256, 0, s>>>(static_cast<uint32_t *>(a.C),
-                                                                                   n / 4, 0u);
-    if (n % 4)
-      fill_kernel<uint8_t><<<1, 64, 0, s>>>(static_cast<uint8_t *>(a.C) + (n & ~3LL), n % 4, (uint8_t)0);
-    BGB_HIP_TRY(hipGetLastError());
-  }
+  void *ws = nullptr;
+  const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
+  int st = stream_workspace(s, key_bytes + sizeof(Acc) * (size_t)nchunks * T, &ws);
+  if (st) return st;
+  a.carry_key = static_cast<int *>(ws);
+  a.carry_val = static_cast<char *>(ws) + key_bytes;
   const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
   bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
+  BGB_HIP_TRY(hipGetLastError());
+  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : nchunks * (T / 4);
+  bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }
@@ -372,7 +407,7 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
     BGB_HIP_TRY(hipGetLastError());
     return BGB_OK;
   }
-  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0};
+  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr};
   switch (T) {
     case 4: return launch_bmv<4, OP>(a, s);
     case 8: return launch_bmv<8, OP>(a, s);

@@ -27,14 +27,16 @@ struct Model {
   double row_ns[4];
   double bw_gbs;
 };
-// op 0: bin_bin_full / bin_bin_bin, op 1: bin_full_full (FP32), op 2: K=16 MFMA
+// op 0: bin_bin_full / bin_bin_bin, op 1: bin_full_full (FP32), op 2: K=16 MFMA,
+// op 3: one level of the batched 64-source BFS
 // Fitted to MI355X runs of the chunked kernels on R-MAT scale 20 / 22 and the
 // 4096^2 grid (whole-chip time per tile at full occupancy; the small tiles are
 // issue/latency bound, T = 32 is bandwidth bound at ~4.2 TB/s).
-const Model kModels[3] = {
+const Model kModels[4] = {
     {{0.0063, 0.0063, 0.0118, 0.0200}, {0.0020, 0.0030, 0.0050, 0.0100}, 4200.0},
     {{0.0250, 0.0300, 0.0600, 0.1500}, {0.0030, 0.0050, 0.0100, 0.0200}, 4000.0},
     {{1e9, 1e9, 0.0600, 0.2000}, {1e9, 1e9, 0.0300, 0.0600}, 4000.0},
+    {{0.0132, 0.0191, 0.0350, 0.0800}, {0.0050, 0.0080, 0.0150, 0.0300}, 4000.0},
 };
 
 double op_bytes(int op, int T, long long nrows, long long nblocks) {
@@ -44,6 +46,7 @@ double op_bytes(int op, int T, long long nrows, long long nblocks) {
   const double rp = 4.0 * (nbr + 1);
   if (op == 0) return tiles + rp + nbr * T * wb / T + 4.0 * nbr * T;  // packed x + int32 y
   if (op == 1) return tiles + rp + 8.0 * nbr * T;                     // FP32 x + y
+  if (op == 3) return tiles + rp + 24.0 * nbr * T;                    // frontier, visited, next
   return tiles + rp + nbr * T * 16 * (2.0 + 4.0);                     // FP16 X + FP32 Y
 }
 
@@ -83,7 +86,7 @@ int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colin
 int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
                          double *cost4_us) {
   bgb::clear_error();
-  BGB_REQUIRE(op >= 0 && op <= 2 && nrows >= 0 && h_nblocks4 && T_out, BGB_ERR_ARG, "bad arguments");
+  BGB_REQUIRE(op >= 0 && op <= 3 && nrows >= 0 && h_nblocks4 && T_out, BGB_ERR_ARG, "bad arguments");
   if (op == 2) allowed_mask &= 0xC;  // the MFMA path takes T = 16, 32
   BGB_REQUIRE(allowed_mask & 0xF, BGB_ERR_ARG, "no tile dim allowed");
   const Model &m = kModels[op];

@@ -84,8 +84,9 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
  * (|= for bits, += for counts and sums); 0 overwrites C.                    *
  * Outputs are 4-byte aligned; bit outputs (W_T[NBR]) are rounded up to a   *
  * whole number of dwords: pieces of rows that cross the kernels' tile      *
- * chunks are OR-ed / added atomically into them (C is zero-filled first    *
- * unless accumulating). Needs no workspace.                                *
+ * chunks are OR-ed / added atomically into them by a second, small pass.   *
+ * Calls on one stream are ordered; calls that overlap on different streams *
+ * use separate per-stream carry workspaces.                                *
  * ------------------------------------------------------------------------ */
 
 /* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
@@ -218,7 +219,8 @@ int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colin
 int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colind,
                    long long *h_nblocks4, long long *h_nonempty4);
 
-/* Picks T for an op family (0: bin_bin_*, 1: bin_full_full, 2: K=16 MFMA)
+/* Picks T for an op family (0: bin_bin_*, 1: bin_full_full, 2: K=16 MFMA,
+ * 3: a level of the batched BFS)
  * from the tile counts with a cost model fitted to MI355X runs of these
  * kernels. allowed_mask bit t enables T = 4 << t. cost4_us (may be NULL)
  * receives the predicted microseconds per T. */
