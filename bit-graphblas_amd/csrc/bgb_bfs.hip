@@ -180,15 +180,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   uint64_t sc[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) sc[i] = acc[i];
-#pragma unroll
-  for (int d = 1; d < G; d <<= 1) {
-    const int ku = __shfl_up(key, d * L);
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const uint64_t up = __shfl_up(sc[i], d * L);
-      if (g >= d && ku == key) sc[i] |= up;
-    }
-  }
+  group_scan<L, R, uint64_t>(key, sc, [](uint64_t x, uint64_t y) { return x | y; });
   const int kin = __shfl_up(key, L);
 #pragma unroll
   for (int i = 0; i < R; ++i) {

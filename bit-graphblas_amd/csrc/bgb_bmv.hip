@@ -264,15 +264,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   Acc sc[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) sc[i] = acc[i];
-#pragma unroll
-  for (int d = 1; d < G; d <<= 1) {
-    const int ku = __shfl_up(key, d * L);
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const Acc up = __shfl_up(sc[i], d * L);
-      if (g >= d && ku == key) sc[i] += up;
-    }
-  }
+  group_scan<L, R, Acc>(key, sc, [](Acc x, Acc y) { return x + y; });
   // Carry into this group's first flushed row from the groups before it.
   const int kin = __shfl_up(key, L);
   Acc cin[R];
@@ -286,16 +278,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
     for (int i = 0; i < R; ++i) hv[i] += cin[i];
     flush_row<T, OP>(a, held, q, hv);
   }
-  // The last active group publishes the chunk's open row, unless its partial
-  // is all zero (then there is nothing to add).
+  // The last active group publishes the chunk's open row. Bit carries are
+  // skipped when all zero (key -1); count / FP carries are always published so
+  // that the chunks carrying into one row stay contiguous for the fix-up.
   const int last = (int)lmin(G - 1, (a.nblocks - 1 - k0) / S);
   if (g == last) {
-    bool nz = false;
+    bool open = key < a.nbr;
+    if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+      bool nz = false;
 #pragma unroll
-    for (int i = 0; i < R; ++i) nz |= sc[i] != (Acc)0;
+      for (int i = 0; i < R; ++i) nz |= sc[i] != (Acc)0;
 #pragma unroll
-    for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
-    const bool open = key < a.nbr && nz;
+      for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
+      open = open && nz;
+    }
     if (q == 0) a.carry_key[chunk] = open ? key : -1;
     if (open) {
       Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
@@ -306,15 +302,24 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 }
 
 // Adds the chunk carries of rows that cross chunk boundaries after the main
-// pass. Chunks with no (non-zero) carry published key -1. Count / FP carries:
-// one thread per 4 consecutive values (one 16-byte load, independent
-// atomics); bit carries: one thread per chunk (one 32-bit OR).
+// pass.
+//
+// Bit carries (skipped when zero, key -1): one thread per chunk, one 32-bit OR
+// into the aligned dword holding the row's word.
+//
+// Count / FP carries: one wave per 64 consecutive chunks, lane j owning chunk
+// c0 + j. Carry keys are non-decreasing over chunks (-1 only past the last
+// block row), so the chunks carrying into one row are contiguous; a segmented
+// scan over the lanes sums each run and the run's last lane adds the sum to
+// the output -- a plain read-modify-write when the run lies inside this
+// wave's 64 chunks (it is then the row's only writer in this pass), atomics
+// when the run continues into a neighbouring wave's chunks.
 template <int T, int OP>
-__global__ void bmv_fixup_kernel(BmvArgs a) {
+__global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.nchunks) return;
     const int key = a.carry_key[tid];
     if (key < 0) return;
@@ -329,20 +334,55 @@ __global__ void bmv_fixup_kernel(BmvArgs a) {
     const int shift = (int)(addr & 3) * 8;
     atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << shift);
   } else {
-    constexpr int Q = T / 4;  // threads per chunk
-    if (tid >= (long long)a.nchunks * Q) return;
-    const long long c = tid / Q;
-    const int r0 = (int)(tid % Q) * 4;
-    const int key = a.carry_key[c];
-    if (key < 0) return;
     typedef Acc Acc4 __attribute__((ext_vector_type(4)));
-    const Acc4 v = *reinterpret_cast<const Acc4 *>(static_cast<const Acc *>(a.carry_val) + c * T + r0);
-    const long long base = (long long)key * T + r0;
+    const int lane = threadIdx.x & 63;
+    const long long c0 = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    if (c0 >= a.nchunks) return;  // uniform over the wave
+    const long long c = c0 + lane;
+    const bool in = c < a.nchunks;
+    int key = in ? a.carry_key[c] : -1;
+    if (key < 0) key = INT_MAX;  // no open row: never written
+    Acc v[T];
+    const Acc4 *cv = reinterpret_cast<const Acc4 *>(static_cast<const Acc *>(a.carry_val) + c * T);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (v[i] == (Acc)0) continue;
-      if constexpr (OP == OP_COUNT_I32) atomicAdd(static_cast<int *>(a.C) + base + i, (int)v[i]);
-      else unsafeAtomicAdd(static_cast<float *>(a.C) + base + i, (float)v[i]);
+    for (int r = 0; r < T / 4; ++r) {
+      const Acc4 f = key != INT_MAX ? cv[r] : Acc4{0, 0, 0, 0};
+      v[4 * r] = f.x;
+      v[4 * r + 1] = f.y;
+      v[4 * r + 2] = f.z;
+      v[4 * r + 3] = f.w;
+    }
+    group_scan<1, T, Acc>(key, v, [](Acc x, Acc y) { return x + y; });
+    const int key0 = __shfl(key, 0), kend = __shfl(key, 63);
+    const int knext = __shfl_down(key, 1);
+    const bool run_end = lane == 63 || knext != key;
+    if (!run_end || key == INT_MAX) return;
+    // does the run continue from the previous / into the next wave's chunks?
+    const bool from_prev = key == key0 && c0 > 0 && a.carry_key[c0 - 1] == key;
+    const bool into_next = lane == 63 && c0 + 64 < a.nchunks && a.carry_key[c0 + 64] == kend;
+    Acc *y = static_cast<Acc *>(a.C) + (long long)key * T;
+    if (from_prev || into_next) {
+#pragma unroll
+      for (int r = 0; r < T; ++r) {
+        if (v[r] == (Acc)0) continue;
+        if constexpr (OP == OP_COUNT_I32) atomicAdd(reinterpret_cast<int *>(y) + r, (int)v[r]);
+        else unsafeAtomicAdd(reinterpret_cast<float *>(y) + r, (float)v[r]);
+      }
+    } else {
+      bool nz = false;
+#pragma unroll
+      for (int r = 0; r < T; ++r) nz |= v[r] != (Acc)0;
+      if (!nz) return;
+      Acc4 *y4 = reinterpret_cast<Acc4 *>(y);
+#pragma unroll
+      for (int r = 0; r < T / 4; ++r) {
+        Acc4 t = y4[r];
+        t.x += v[4 * r];
+        t.y += v[4 * r + 1];
+        t.z += v[4 * r + 2];
+        t.w += v[4 * r + 3];
+        y4[r] = t;
+      }
     }
   }
 }
@@ -369,7 +409,8 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
   bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
-  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : nchunks * (T / 4);
+  // bits: one thread per chunk; count / FP: one wave per 64 chunks
+  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : ceil_div(nchunks, 64) * 64;
   bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;

@@ -204,6 +204,80 @@ __device__ __forceinline__ int groups_find_row(const int *__restrict__ rowptr, i
   return active ? group_find_row(rowptr, Ia, Ib, pa) : Ia;
 }
 
+// Value of lane (lane - D) inside the lane's 16-lane row (DPP row_shr:D, a
+// VALU operand modifier, no LDS round trip); lanes with (lane % 16) < D get
+// `fill`.
+template <int D>
+__device__ __forceinline__ uint32_t dpp_shr(uint32_t v, uint32_t fill) {
+  static_assert(D >= 1 && D < 16, "row_shr distance");
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x110 + D, 0xF, 0xF, false);
+}
+
+// Value of lane (lane - D) across the wave (lanes < D keep their own value),
+// DPP for in-row distances, ds_bpermute otherwise.
+template <int D>
+__device__ __forceinline__ uint32_t lane_up(uint32_t v, uint32_t fill) {
+  if constexpr (D < 16) return dpp_shr<D>(v, fill);
+  else return (uint32_t)__shfl_up((int)v, D);
+}
+template <int D>
+__device__ __forceinline__ float lane_up(float v, float fill) {
+  return __uint_as_float(lane_up<D>(__float_as_uint(v), __float_as_uint(fill)));
+}
+template <int D>
+__device__ __forceinline__ uint64_t lane_up(uint64_t v, uint64_t fill) {
+  const uint32_t lo = lane_up<D>((uint32_t)v, (uint32_t)fill);
+  const uint32_t hi = lane_up<D>((uint32_t)(v >> 32), (uint32_t)(fill >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive segmented scan over lane groups of L lanes (L divides 16; keys
+// non-decreasing over groups, so equal keys are contiguous): afterwards sc[i]
+// of group g holds the combination of groups g' <= g with key(g') == key(g).
+// Hillis-Steele inside each 16-lane row with DPP row_shr (lanes whose source
+// lies in an earlier row read the sentinel key and keep their value), then two
+// steps across rows: row r takes the last group of row r-1, then row r >= 2
+// the last group of row r-2 (covering rows r-3..r-2 by then). Keys are never INT_MIN.
+// Must be called by all 64 lanes.
+template <int L, int R, class Acc, class Op>
+__device__ __forceinline__ void group_scan(int key, Acc (&sc)[R], Op op) {
+  static_assert(16 % L == 0, "groups must tile the 16-lane rows");
+  constexpr uint32_t kNone = 0x80000000u;
+#define BGB_ROW_STEP(DL)                                                       \
+  if constexpr ((DL) < 16) {                                                   \
+    const int ku = (int)dpp_shr<(DL)>((uint32_t)key, kNone);                    \
+    _Pragma("unroll") for (int i = 0; i < R; ++i) {                            \
+      const Acc up = lane_up<(DL)>(sc[i], (Acc)0);                             \
+      if (ku == key) sc[i] = op(sc[i], up);                                    \
+    }                                                                          \
+  }
+  BGB_ROW_STEP(L)
+  BGB_ROW_STEP(2 * L)
+  BGB_ROW_STEP(4 * L)
+  BGB_ROW_STEP(8 * L)
+#undef BGB_ROW_STEP
+  const int lane = threadIdx.x & 63, row = lane >> 4;
+  // last group of row r-1, same lane position inside the group
+  const int src1 = row * 16 - L + (lane & (L - 1));
+  {
+    const int ku = __shfl(key, src1 & 63);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const Acc up = __shfl(sc[i], src1 & 63);
+      if (row >= 1 && ku == key) sc[i] = op(sc[i], up);
+    }
+  }
+  const int src2 = row * 16 - 16 - L + (lane & (L - 1));  // last group of row r-2
+  {
+    const int ku = __shfl(key, src2 & 63);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const Acc up = __shfl(sc[i], src2 & 63);
+      if (row >= 2 && ku == key) sc[i] = op(sc[i], up);
+    }
+  }
+}
+
 // Workgroup index remapped so that consecutive index ranges stay on one XCD
 // (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
 __device__ __forceinline__ int xcd_remap(int b, int n) {
