@@ -59,6 +59,7 @@ struct BmvArgs {
   int nchunks;
   int *carry_key;
   void *carry_val;
+  const int *part;  // part[c]: block row of chunk c's first tile
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -122,9 +123,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
+  __shared__ int s_win[kWavesPerBlock][kWin];
   const int lane = threadIdx.x & 63;
-  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                                                   (threadIdx.x >> 6));
   if (chunk >= a.nchunks) return;  // uniform per wave
+  // rows of this chunk's first tile and of the next chunk's (scalar loads)
+  const int Ia = a.part[chunk];
+  const int Ib = chunk + 1 < a.nchunks ? a.part[chunk + 1] : a.nbr - 1;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
@@ -180,9 +186,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
     }
   }
 
-  // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1].
-  const int2 ab = wave_find_rows2(rowptr, a.nbr - 1, k0, lmin(k0 + CH, a.nblocks) - 1, lane);
-  int I = groups_find_row(rowptr, ab.x, ab.y, pa, active, lane);
+  // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1], searched in
+  // the wave's staged row ends.
+  RowWindow rw;
+  rw.stage(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane);
+  int I = active ? rw.find(pa) : Ia;
   const int I0 = I;
 
   Acc acc[R], zero[R];
@@ -197,7 +205,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   Acc hv[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) hv[i] = (Acc)0;
-  int e = active ? rowptr[I + 1] : INT_MAX;
+  int e = active ? rw.at(I + 1) : INT_MAX;
 
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -248,11 +256,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
         ++I;
-        e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
+        e = I < a.nbr ? rw.at(I + 1) : INT_MAX;
         while (e == pos) {
           flush_row<T, OP>(a, I, q, zero);
           ++I;
-          e = I < a.nbr ? rowptr[I + 1] : INT_MAX;
+          e = I < a.nbr ? rw.at(I + 1) : INT_MAX;
         }
       }
     }
@@ -360,27 +368,30 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
     // does the run continue from the previous / into the next wave's chunks?
     const bool from_prev = key == key0 && c0 > 0 && a.carry_key[c0 - 1] == key;
     const bool into_next = lane == 63 && c0 + 64 < a.nchunks && a.carry_key[c0 + 64] == kend;
-    Acc *y = static_cast<Acc *>(a.C) + (long long)key * T;
+    // output element type: int32 counts, else float (counts converted)
+    using Out = typename std::conditional<OP == OP_COUNT_I32, int, float>::type;
+    typedef Out Out4 __attribute__((ext_vector_type(4)));
+    Out *y = static_cast<Out *>(a.C) + (long long)key * T;
     if (from_prev || into_next) {
 #pragma unroll
       for (int r = 0; r < T; ++r) {
         if (v[r] == (Acc)0) continue;
-        if constexpr (OP == OP_COUNT_I32) atomicAdd(reinterpret_cast<int *>(y) + r, (int)v[r]);
-        else unsafeAtomicAdd(reinterpret_cast<float *>(y) + r, (float)v[r]);
+        if constexpr (OP == OP_COUNT_I32) atomicAdd(y + r, (int)v[r]);
+        else unsafeAtomicAdd(y + r, (float)v[r]);
       }
     } else {
       bool nz = false;
 #pragma unroll
       for (int r = 0; r < T; ++r) nz |= v[r] != (Acc)0;
       if (!nz) return;
-      Acc4 *y4 = reinterpret_cast<Acc4 *>(y);
+      Out4 *y4 = reinterpret_cast<Out4 *>(y);
 #pragma unroll
       for (int r = 0; r < T / 4; ++r) {
-        Acc4 t = y4[r];
-        t.x += v[4 * r];
-        t.y += v[4 * r + 1];
-        t.z += v[4 * r + 2];
-        t.w += v[4 * r + 3];
+        Out4 t = y4[r];
+        t.x += (Out)v[4 * r];
+        t.y += (Out)v[4 * r + 1];
+        t.z += (Out)v[4 * r + 2];
+        t.w += (Out)v[4 * r + 3];
         y4[r] = t;
       }
     }
@@ -402,10 +413,15 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   a.nchunks = (int)nchunks;
   void *ws = nullptr;
   const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
-  int st = stream_workspace(s, key_bytes + sizeof(Acc) * (size_t)nchunks * T, &ws);
+  const size_t val_bytes = (sizeof(Acc) * (size_t)nchunks * T + 255) & ~(size_t)255;
+  int st = stream_workspace(s, key_bytes + val_bytes + sizeof(int) * (size_t)nchunks, &ws);
   if (st) return st;
   a.carry_key = static_cast<int *>(ws);
   a.carry_val = static_cast<char *>(ws) + key_bytes;
+  a.part = reinterpret_cast<int *>(static_cast<char *>(ws) + key_bytes + val_bytes);
+  chunk_rows_kernel<<<(unsigned)ceil_div(a.nbr, 256), 256, 0, s>>>(a.rowptr, a.nbr, CH, a.nchunks,
+                                                                    const_cast<int *>(a.part));
+  BGB_HIP_TRY(hipGetLastError());
   const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
   bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
@@ -448,7 +464,7 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
     BGB_HIP_TRY(hipGetLastError());
     return BGB_OK;
   }
-  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr};
+  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr};
   switch (T) {
     case 4: return launch_bmv<4, OP>(a, s);
     case 8: return launch_bmv<8, OP>(a, s);

@@ -204,6 +204,81 @@ __device__ __forceinline__ int groups_find_row(const int *__restrict__ rowptr, i
   return active ? group_find_row(rowptr, Ia, Ib, pa) : Ia;
 }
 
+// part[c] = block row holding tile c*CH, for every chunk c < nchunks (each
+// chunk's first tile lies in exactly one non-empty row). One lane per block
+// row writes the chunks starting in its row; rows where more than 8 chunks
+// start (hubs) are written by the whole wave together.
+template <int Unused = 0>  // a template: every translation unit that launches it gets the kernel
+__global__ __launch_bounds__(256) void chunk_rows_kernel(const int *__restrict__ rowptr, int nbr, int CH,
+                                                         int nchunks, int *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long c0 = 0, c1 = 0;  // chunks [c0, c1) start in row I
+  if (I < nbr) {
+    const long long r0 = rowptr[I], r1 = rowptr[I + 1];
+    c0 = (r0 + CH - 1) / CH;
+    c1 = (r1 + CH - 1) / CH;
+    if (c1 > nchunks) c1 = nchunks;
+    if (c1 < c0) c1 = c0;
+  }
+  const bool big = c1 - c0 > 8;
+  if (!big)
+    for (long long c = c0; c < c1; ++c) part[c] = (int)I;
+  unsigned long long m = __ballot(big);
+  while (m) {  // wave-uniform
+    const int src = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    const long long b0 = __shfl(c0, src), b1 = __shfl(c1, src);
+    const int row = __shfl((int)I, src);
+    for (long long c = b0 + lane; c < b1; c += 64) part[c] = row;
+  }
+}
+
+// The row ends a wave needs, staged in LDS: rowptr[Ia .. Ib+1], where Ia is
+// the row of the wave's first tile and Ib bounds the row of the first tile
+// after its chunk. When they do not fit (a chunk spanning more than kWin-2
+// block rows, e.g. long runs of empty rows) lookups go to memory.
+constexpr int kWin = 256;
+struct RowWindow {
+  const int *__restrict__ rowptr;
+  int *win;
+  int Ia, Ib;
+  bool staged;  // wave-uniform
+
+  // Must be called by all 64 lanes.
+  __device__ __forceinline__ void stage(const int *__restrict__ rp, int *w, int ia, int ib, int lane) {
+    rowptr = rp;
+    win = w;
+    Ia = ia;
+    Ib = ib;
+    const int m = ib + 2 - ia;
+    staged = m <= kWin;
+    if (staged) {
+#pragma unroll
+      for (int j = 0; j < kWin / 64; ++j) {
+        const int i = lane + 64 * j;
+        if (i < m) win[i] = rowptr[ia + i];
+      }
+    }
+    // the wave reads entries other lanes wrote (wave-private LDS, in order)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // rowptr[i] for Ia <= i <= Ib + 1
+  __device__ __forceinline__ int at(int i) const { return staged ? win[i - Ia] : rowptr[i]; }
+  // Row of tile p (rowptr[I] <= p < rowptr[I+1]) for p inside the chunk.
+  __device__ __forceinline__ int find(long long p) const {
+    int lo = Ia, hi = Ib;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (at(mid) <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+};
+
 // Value of lane (lane - D) inside the lane's 16-lane row (DPP row_shr:D, a
 // VALU operand modifier, no LDS round trip); lanes with (lane % 16) < D get
 // `fill`.
