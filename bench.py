@@ -256,6 +256,9 @@ def main():
     ap.add_argument("--bfs-tile", type=int, default=0, help="tile dim for the BFS section (0: autotune)")
     ap.add_argument("--run-length", type=int, default=0,
                     help="tiles per lane group in the BMV kernels (8 or 16; 0 keeps the library default)")
+    ap.add_argument("--x-staging", type=int, default=-1,
+                    help="packed-x staging of the bit BMVs for every tile dim: 0 auto, 1 never (gathered "
+                         "from memory), 2 in LDS whenever it fits; -1 keeps the library default")
     ap.add_argument("--profile", action="store_true",
                     help="profiling workload only: a 256 MiB copy (counter calibration) then --steps BMVs "
                          "at the first of --tiles")
@@ -277,6 +280,10 @@ def main():
         for T in (4, 8, 16, 32):
             bgb.call("bgb_set_run_length", T, args.run_length)
     run_lengths = {T: bgb.lib().bgb_get_run_length(T) for T in (4, 8, 16, 32)}
+    if args.x_staging >= 0:
+        for T in (4, 8, 16, 32):
+            bgb.call("bgb_set_x_staging", T, args.x_staging)
+    x_staging = {T: bgb.lib().bgb_get_x_staging(T) for T in (4, 8, 16, 32)}
     cfg = CONFIGS[args.config]
     log(f"[bench] generating R-MAT scale {cfg['scale']} ef {cfg['edge_factor']}")
     t0 = time.perf_counter()
@@ -393,7 +400,7 @@ def main():
             "workload": (f"bin_bin_full BMV, R-MAT scale {cfg['scale']} ef {cfg['edge_factor']} "
                          "(a=.57 b=.19 c=.19) symmetrised, x Bernoulli(0.05), best tile dim"),
             "tile_dim": best, "n": csr.nrows, "nnz": csr.nnz, "nblocks": b["nblocks"],
-            "run_length": run_lengths[best],
+            "run_length": run_lengths[best], "x_staging": x_staging[best],
         },
         "roofline": {"bound": "hbm", "achieved": round(b["alg_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4), "traffic": None},

@@ -41,6 +41,8 @@ SIGNATURES = {
     "bgb_pack_vec": (_c_int, [_c_int, _vp, _c_int, _vp, _vp]),
     "bgb_set_run_length": (_c_int, [_c_int, _c_int]),
     "bgb_get_run_length": (_c_int, [_c_int]),
+    "bgb_set_x_staging": (_c_int, [_c_int, _c_int]),
+    "bgb_get_x_staging": (_c_int, [_c_int]),
     "bgb_bmv_bin_bin_bin": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_full_i32": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_full_f32": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),

@@ -55,6 +55,7 @@ struct BfsState {
   int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done
   int *carry_key;
   uint64_t *carry_val;
+  const int *part;       // part[c]: block row of chunk c's first tile
 };
 
 // Records the vertices first reached at this level (new bits) for one row.
@@ -102,11 +103,15 @@ template <int T, int S>
 __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
+  __shared__ int s_win[kBfsWaves][kWin];
   if (b.ctl[2]) return;  // BFS already finished: this captured level is a no-op
   const int depth = b.ctl[0];
   const int lane = threadIdx.x & 63;
-  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves + (threadIdx.x >> 6);
+  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves +
+                                                   (threadIdx.x >> 6));
   if (chunk >= b.nchunks) return;
+  const int Ia = b.part[chunk];
+  const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
@@ -120,8 +125,9 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   uint32_t X[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(b.fnz)[J[s]] : 0u;
-  const int2 ab = wave_find_rows2(rowptr, b.nbr - 1, k0, lmin2(k0 + CH, b.nblocks) - 1, lane);
-  int I = groups_find_row(rowptr, ab.x, ab.y, pa, active, lane);
+  RowWindow rw;
+  rw.stage(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane);
+  int I = active ? rw.find(pa) : Ia;
   const int I0 = I;
 
   bool any = false;
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   if (active) load_visited<T>(b, I, q, vr);
   int held = -1;
-  int e = active ? rowptr[I + 1] : INT_MAX;
+  int e = active ? rw.at(I + 1) : INT_MAX;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (s < n) {
@@ -165,12 +171,12 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
         ++I;
-        e = I < b.nbr ? rowptr[I + 1] : INT_MAX;
+        e = I < b.nbr ? rw.at(I + 1) : INT_MAX;
         while (e == pos) {
           load_visited<T>(b, I, q, vr);
           bfs_flush<T>(b, I, q, zero, vr, depth, any);
           ++I;
-          e = I < b.nbr ? rowptr[I + 1] : INT_MAX;
+          e = I < b.nbr ? rw.at(I + 1) : INT_MAX;
         }
         if (I < b.nbr) load_visited<T>(b, I, q, vr);
       }
@@ -312,7 +318,7 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   const long long nwords = ceil_div(nbr, T) * T;
   const int wb = word_bytes(T);
   const long long nchunks = ceil_div(nblocks, CH);
-  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv;
+  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv, pt;
   BGB_HIP_TRY(F0.alloc(8 * npad, s));
   BGB_HIP_TRY(F1.alloc(8 * npad, s));
   BGB_HIP_TRY(Vb.alloc(8 * npad, s));
@@ -323,6 +329,9 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   BGB_HIP_TRY(src.alloc(sizeof(int) * nsrc, s));
   BGB_HIP_TRY(ck.alloc(sizeof(int) * (nchunks + 1), s));
   BGB_HIP_TRY(cv.alloc(8 * (nchunks + 1) * T, s));
+  BGB_HIP_TRY(pt.alloc(sizeof(int) * (nchunks + 1), s));
+  if (nchunks > 0)  // chunk -> first block row, once for every level
+    chunk_rows_kernel<<<nblk(nbr), 256, 0, s>>>(rowptr, nbr, CH, (int)nchunks, pt.as<int>());
   const uint64_t full = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
   bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F0.as<uint64_t>(), npad, 0ull);
   bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F1.as<uint64_t>(), npad, 0ull);
@@ -359,6 +368,7 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     b.ctl = ctl.as<int>();
     b.carry_key = ck.as<int>();
     b.carry_val = cv.as<uint64_t>();
+    b.part = pt.as<int>();
   }
   const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
   const unsigned fblocks = nblk(nchunks * T);

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 #include <climits>
 #include <type_traits>
 #include <map>
@@ -59,7 +60,7 @@ struct BmvArgs {
   int nchunks;
   int *carry_key;
   void *carry_val;
-  const int *part;  // part[c]: block row of chunk c's first tile
+  int *part;  // part[c]: block row of chunk c's first tile (cached per stream and rowptr)
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -68,6 +69,8 @@ constexpr int kWavesPerBlock = 4;
 // Longer runs put more bytes in flight per wave, which matters for the small
 // tiles; set with bgb_set_run_length.
 int g_run_length[4] = {8, 8, 8, 8};
+// Packed-x staging per tile dim: 0 automatic, 1 never, 2 whenever it fits.
+int g_x_staging[4] = {0, 0, 0, 0};
 
 template <int OP>
 struct AccOf {
@@ -118,21 +121,28 @@ __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
   }
 }
 
-template <int T, int OP, int S>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
+// Packed-x word J of a copy staged in LDS: T = 4 keeps two nibbles per byte
+// there (word J in byte J/2, high nibble for odd J), T >= 8 the words as
+// they are.
+template <int T>
+__device__ __forceinline__ uint32_t lds_xword(const uint8_t *xl, int J) {
+  if constexpr (T == 4) return (uint32_t)(xl[J >> 1] >> ((J & 1) * 4)) & 0xFu;
+  else return (uint32_t) reinterpret_cast<const typename Tile<T>::W *>(xl)[J];
+}
+
+// One chunk of CH tiles for one wave (all 64 lanes): `win` is the wave's LDS
+// row window, `xl` the packed x staged in LDS (XL) or unused (x gathered from
+// memory).
+template <int T, int OP, int S, bool XL>
+__device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane, int *win, const uint8_t *xl) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
-  __shared__ int s_win[kWavesPerBlock][kWin];
-  const int lane = threadIdx.x & 63;
-  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock +
-                                                   (threadIdx.x >> 6));
-  if (chunk >= a.nchunks) return;  // uniform per wave
-  // rows of this chunk's first tile and of the next chunk's (scalar loads)
-  const int Ia = a.part[chunk];
-  const int Ib = chunk + 1 < a.nchunks ? a.part[chunk + 1] : a.nbr - 1;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
+  // rows of this chunk's first tile and of the next chunk's
+  const int2 ab = chunk_row_range(a.rowptr, a.part, chunk, a.nchunks, a.nbr, k0, k0 + CH, lane);
+  const int Ia = ab.x, Ib = ab.y;
   const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
   const long long pb = lmin(pa + S, a.nblocks);
   const bool active = pa < pb;
@@ -147,7 +157,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   uint32_t X[S];
   if constexpr (!is_full(OP)) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
+    for (int s = 0; s < S; ++s) {
+      if constexpr (XL) X[s] = (active && s < n) ? lds_xword<T>(xl, J[s]) : 0u;
+      else X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
+    }
   }
 
   // FP ops with T <= 8: the whole x segment of each tile (T values) is loaded
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1], searched in
   // the wave's staged row ends.
   RowWindow rw;
-  rw.stage(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane);
+  rw.stage(rowptr, win, Ia, Ib, lane);
   int I = active ? rw.find(pa) : Ia;
   const int I0 = I;
 
@@ -309,6 +322,60 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   }
 }
 
+template <int T, int OP, int S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
+  __shared__ int s_win[kWavesPerBlock][kWin];
+  const int lane = threadIdx.x & 63;
+  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                                                   (threadIdx.x >> 6));
+  if (chunk >= a.nchunks) return;  // uniform per wave
+  bmv_chunk<T, OP, S, false>(a, chunk, lane, s_win[threadIdx.x >> 6], nullptr);
+}
+
+// Bit-vector ops with the whole packed x in LDS (it fits for n up to ~1.1M):
+// one workgroup of kLdsWaves waves per CU (or two, if they fit) stages x
+// once, then each wave walks chunks gw, gw + W, ... (W waves in the grid).
+// The per-tile x gathers become LDS reads instead of one L2 request each.
+constexpr int kLdsWaves = 16;
+template <int T, int OP, int S>
+__global__ __launch_bounds__(64 * kLdsWaves) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
+  extern __shared__ __align__(16) uint8_t s_dyn[];
+  int *win_all = reinterpret_cast<int *>(s_dyn);
+  uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWin;
+  // stage the packed x (x_bytes bytes, 16-byte aligned): whole 16-byte
+  // units, then the tail byte by byte
+  const uint4 *src = static_cast<const uint4 *>(a.B);
+  const uint8_t *srcb = static_cast<const uint8_t *>(a.B);
+  const int units = x_bytes / 16;
+  if constexpr (T == 4) {
+    // two words (low nibbles) per LDS byte: 16 source bytes -> 8 LDS bytes
+    auto pack = [](uint32_t w) {  // bytes b0..b3 -> (b0 | b1 << 4) | (b2 | b3 << 4) << 8
+      return (w & 0xFu) | ((w >> 4) & 0xF0u) | ((w >> 8) & 0xF00u) | ((w >> 12) & 0xF000u);
+    };
+    uint2 *dst = reinterpret_cast<uint2 *>(xl);
+    for (int i = threadIdx.x; i < units; i += blockDim.x) {
+      const uint4 v = src[i];
+      dst[i] = make_uint2(pack(v.x) | (pack(v.y) << 16), pack(v.z) | (pack(v.w) << 16));
+    }
+    for (int k = units * 8 + threadIdx.x; 2 * k < x_bytes; k += blockDim.x) {
+      const uint32_t lo = srcb[2 * k] & 0xFu;
+      const uint32_t hi = 2 * k + 1 < x_bytes ? (srcb[2 * k + 1] & 0xFu) : 0u;
+      xl[k] = (uint8_t)(lo | (hi << 4));
+    }
+  } else {
+    uint4 *dst = reinterpret_cast<uint4 *>(xl);
+    for (int i = threadIdx.x; i < units; i += blockDim.x) dst[i] = src[i];
+    for (int k = units * 16 + threadIdx.x; k < x_bytes; k += blockDim.x) xl[k] = srcb[k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = gridDim.x * kLdsWaves;
+  for (int c = (int)blockIdx.x * kLdsWaves + wave; c < a.nchunks; c += nw) {
+    const int chunk = __builtin_amdgcn_readfirstlane(c);  // wave-uniform: scalar loads of part / rowptr
+    bmv_chunk<T, OP, S, true>(a, chunk, lane, win_all + wave * kWin, xl);
+  }
+}
+
 // Adds the chunk carries of rows that cross chunk boundaries after the main
 // pass.
 //
@@ -414,17 +481,44 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   void *ws = nullptr;
   const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
   const size_t val_bytes = (sizeof(Acc) * (size_t)nchunks * T + 255) & ~(size_t)255;
-  int st = stream_workspace(s, key_bytes + val_bytes + sizeof(int) * (size_t)nchunks, &ws);
+  int st = stream_workspace(s, key_bytes + val_bytes, &ws);
   if (st) return st;
   a.carry_key = static_cast<int *>(ws);
   a.carry_val = static_cast<char *>(ws) + key_bytes;
-  a.part = reinterpret_cast<int *>(static_cast<char *>(ws) + key_bytes + val_bytes);
-  chunk_rows_kernel<<<(unsigned)ceil_div(a.nbr, 256), 256, 0, s>>>(a.rowptr, a.nbr, CH, a.nchunks,
-                                                                    const_cast<int *>(a.part));
-  BGB_HIP_TRY(hipGetLastError());
-  const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
-  bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
-  BGB_HIP_TRY(hipGetLastError());
+  if ((st = chunk_partition(s, a.rowptr, a.nbr, a.nblocks, CH, &a.part))) return st;
+  bool lds_path = false;
+  if constexpr (!is_full(OP)) {
+    // Whole packed x in LDS when it fits and the tile stream is large next to
+    // staging it once per workgroup.
+    DeviceFacts f;
+    if ((st = device_facts(&f))) return st;
+    const long long x_bytes = ceil_div(a.nbr, T) * T * (long long)sizeof(typename Tile<T>::W);
+    const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;
+    const long long lds = (long long)sizeof(int) * kLdsWaves * kWin + xl;
+    const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(typename Tile<T>::W));
+    const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
+    const int mode = g_x_staging[ilog2_tiledim(T) - 2];
+    const bool worth = mode == 2 || (mode == 0 && tile_bytes >= 2 * xl * f.cus);
+    if (mode != 1 && aligned && lds <= f.lds_per_block && worth) {
+      static bool attr_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S>),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)f.lds_per_block) == hipSuccess;
+      (void)hipGetLastError();
+      if (attr_ok) {
+        const int per_cu = std::max(1, std::min(2, f.lds_per_cu / (int)lds));
+        const long long want = ceil_div(nchunks, kLdsWaves);
+        const unsigned grid = (unsigned)std::min<long long>((long long)f.cus * per_cu, want);
+        bmv_lds_kernel<T, OP, S><<<grid, 64 * kLdsWaves, (size_t)lds, s>>>(a, (int)x_bytes);
+        BGB_HIP_TRY(hipGetLastError());
+        lds_path = true;
+      }
+    }
+  }
+  if (!lds_path) {
+    const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
+    bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
+    BGB_HIP_TRY(hipGetLastError());
+  }
   // bits: one thread per chunk; count / FP: one wave per 64 chunks
   const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : ceil_div(nchunks, 64) * 64;
   bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
@@ -509,6 +603,92 @@ int stream_workspace(hipStream_t s, size_t bytes, void **out) {
   return BGB_OK;
 }
 
+namespace {
+std::mutex g_facts_mu;
+std::map<int, DeviceFacts> g_facts;
+}  // namespace
+
+int device_facts(DeviceFacts *out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_facts_mu);
+  auto it = g_facts.find(dev);
+  if (it == g_facts.end()) {
+    DeviceFacts f;
+    BGB_HIP_TRY(hipDeviceGetAttribute(&f.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BGB_HIP_TRY(hipDeviceGetAttribute(&f.lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    BGB_HIP_TRY(hipDeviceGetAttribute(&f.lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    if (f.lds_per_cu < f.lds_per_block) f.lds_per_cu = f.lds_per_block;
+    it = g_facts.emplace(dev, f).first;
+  }
+  *out = it->second;
+  return BGB_OK;
+}
+
+// Chunk partitions (chunk -> block row of its first tile), one per (device,
+// stream, rowptr address, nbr, nblocks, chunk size), kept for the next call on
+// the same matrix. The kernels check every entry against rowptr and repair
+// it, so a caller that rewrites rowptr in place only costs a search.
+namespace {
+struct PartEntry {
+  int dev;
+  hipStream_t s;
+  const int *rowptr;
+  int nbr, nblocks, CH;
+  int *part;
+  size_t cap;  // ints
+};
+constexpr int kPartCache = 8;
+std::mutex g_part_mu;
+std::vector<PartEntry> g_part;  // most recently used last
+}  // namespace
+
+int chunk_partition(hipStream_t s, const int *rowptr, int nbr, int nblocks, int CH, int **out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  const long long nchunks = ceil_div(nblocks, CH);
+  std::lock_guard<std::mutex> lk(g_part_mu);
+  for (size_t i = 0; i < g_part.size(); ++i) {
+    PartEntry &e = g_part[i];
+    if (e.dev == dev && e.s == s && e.rowptr == rowptr && e.nbr == nbr && e.nblocks == nblocks && e.CH == CH) {
+      PartEntry hit = e;
+      g_part.erase(g_part.begin() + (long)i);
+      g_part.push_back(hit);
+      *out = hit.part;
+      return BGB_OK;
+    }
+  }
+  PartEntry ne{dev, s, rowptr, nbr, nblocks, CH, nullptr, 0};
+  // cache full: evict the least recently used entry. Queued work may still
+  // read it; work on this stream is ordered before the new writes, any other
+  // stream (possibly destroyed by now) is waited for through the device.
+  if ((int)g_part.size() >= kPartCache) {
+    PartEntry old = g_part.front();
+    g_part.erase(g_part.begin());
+    if (old.dev == dev && old.s == s && old.cap >= (size_t)nchunks) {
+      ne.part = old.part;
+      ne.cap = old.cap;
+    } else {
+      BGB_HIP_TRY(hipSetDevice(old.dev));
+      BGB_HIP_TRY(hipDeviceSynchronize());
+      BGB_HIP_TRY(hipFree(old.part));
+      BGB_HIP_TRY(hipSetDevice(dev));
+    }
+  }
+  if (!ne.part) {
+    ne.cap = (size_t)std::max<long long>(nchunks, 1);
+    hipError_t e = hipMalloc(&ne.part, sizeof(int) * ne.cap);
+    if (e != hipSuccess)
+      return set_error(BGB_ERR_ALLOC, "partition hipMalloc(%zu) failed: %s", sizeof(int) * ne.cap,
+                       hipGetErrorString(e));
+  }
+  chunk_rows_kernel<<<(unsigned)ceil_div(nbr, 256), 256, 0, s>>>(rowptr, nbr, CH, (int)nchunks, ne.part);
+  BGB_HIP_TRY(hipGetLastError());
+  g_part.push_back(ne);
+  *out = ne.part;
+  return BGB_OK;
+}
+
 }  // namespace bgb
 
 using namespace bgb;
@@ -524,6 +704,16 @@ int bgb_set_run_length(int T, int S) {
 }
 
 int bgb_get_run_length(int T) { return valid_tiledim(T) ? g_run_length[ilog2_tiledim(T) - 2] : -1; }
+
+int bgb_set_x_staging(int T, int mode) {
+  clear_error();
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "x staging mode must be 0, 1 or 2 (got %d)", mode);
+  g_x_staging[ilog2_tiledim(T) - 2] = mode;
+  return BGB_OK;
+}
+
+int bgb_get_x_staging(int T) { return valid_tiledim(T) ? g_x_staging[ilog2_tiledim(T) - 2] : -1; }
 
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C, const int *d_rowptr,
                         const int *d_colind, int nblockrows, int nblocks, int accumulate,

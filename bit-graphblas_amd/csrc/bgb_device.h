@@ -39,6 +39,18 @@ hipError_t exclusive_scan_u8_to_i64(const uint8_t *in, long long *out, size_t n,
 // Per-(device, stream) grow-only scratch used by the BMV carry records. The
 // pointer stays valid until a later call on the same stream asks for more.
 int stream_workspace(hipStream_t s, size_t bytes, void **out);
+// Chunk -> first block row for chunks of CH tiles, cached per (device, stream,
+// rowptr address, sizes); the kernels validate and repair entries.
+int chunk_partition(hipStream_t s, const int *rowptr, int nbr, int nblocks, int CH, int **out);
+
+// Per-device launch facts, queried once per device: compute units and the
+// LDS bytes a workgroup / a CU may use.
+struct DeviceFacts {
+  int cus = 0;
+  int lds_per_block = 0;
+  int lds_per_cu = 0;
+};
+int device_facts(DeviceFacts *out);
 
 inline int ilog2_tiledim(int T) { return T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : 5; }
 

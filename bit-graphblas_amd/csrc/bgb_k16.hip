@@ -189,26 +189,36 @@ struct K16BatchArgs {
   int *graph_of;
 };
 
+// Persistent launch over many graphs. Thread 0 of a workgroup takes one
+// ticket of kWaves chunks from the global queue; the ticket reaches the
+// workgroup's waves through LDS between two barriers, so every wave runs the
+// loop in uniform control flow (no lane-0-only region feeds a wave-wide
+// shuffle) and the whole workgroup leaves the loop together once the queue
+// is drained.
 __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArgs a) {
-  const int lane = threadIdx.x & 63;
+  __shared__ unsigned long long s_ticket;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
-    unsigned long long c = 0;
-    if (lane == 0) c = atomicAdd(a.queue, 1ull);
-    c = __shfl(c, 0);
-    if (c >= (unsigned long long)a.total) break;  // uniform per wave: every wave exits once drained
-    int lo = 0, hi = a.ngraphs - 1;  // graph with chunk_start[g] <= c < chunk_start[g+1]
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.chunk_start[mid] <= (long long)c) lo = mid;
-      else hi = mid - 1;
-    }
-    const K16Graph G = a.graphs[lo];
-    const long long k0 = ((long long)c - a.chunk_start[lo]) * kChunk;
-    const long long k1 = k0 + kChunk < G.nblocks ? k0 + kChunk : G.nblocks;
-    float *carry = a.carry_val + (long long)c * 32 * kK;
-    const int open = a.tdim[lo] == 32 ? k16_chunk<32>(G, k0, k1, lane, carry)
-                                      : k16_chunk<16>(G, k0, k1, lane, carry);
-    if (lane == 0) {
+    if (threadIdx.x == 0) s_ticket = atomicAdd(a.queue, (unsigned long long)kWaves);
+    __syncthreads();
+    const unsigned long long base = s_ticket;
+    __syncthreads();  // every thread has read the ticket before the next one is taken
+    if (base >= (unsigned long long)a.total) break;  // uniform per workgroup
+    const unsigned long long c = base + (unsigned long long)wave;
+    if (c < (unsigned long long)a.total) {  // uniform per wave
+      int lo = 0, hi = a.ngraphs - 1;  // graph with chunk_start[g] <= c < chunk_start[g+1]
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.chunk_start[mid] <= (long long)c) lo = mid;
+        else hi = mid - 1;
+      }
+      const K16Graph G = a.graphs[lo];
+      const long long k0 = ((long long)c - a.chunk_start[lo]) * kChunk;
+      const long long k1 = k0 + kChunk < G.nblocks ? k0 + kChunk : G.nblocks;
+      float *carry = a.carry_val + (long long)c * 32 * kK;
+      const int open = a.tdim[lo] == 32 ? k16_chunk<32>(G, k0, k1, lane, carry)
+                                        : k16_chunk<16>(G, k0, k1, lane, carry);
+      // written by every lane (same value): no lane-0-only region in the loop
       a.carry_key[c] = open < G.nbr ? open : -1;
       a.graph_of[c] = lo;
     }

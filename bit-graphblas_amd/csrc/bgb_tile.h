@@ -39,6 +39,13 @@ struct Tile<32> {
   static constexpr int L = 8, R = 4, NW = 4, LG = 5;
 };
 
+// 16-byte non-temporal load (streamed data that is read once per pass).
+__device__ __forceinline__ uint4 load_nt16(const void *p) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 v = __builtin_nontemporal_load(static_cast<const u32x4 *>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Row i (0 <= i < R) of a lane fragment, as a T-bit word.
 template <int T>
 __device__ __forceinline__ uint32_t row_word(const uint32_t (&F)[Tile<T>::NW], int i) {
@@ -60,7 +67,11 @@ __device__ __forceinline__ void load_frag(const void *__restrict__ A, long long 
     F[1] = v.y;
   } else {
     constexpr int L = Tile<T>::L;
-    const uint4 v = static_cast<const uint4 *>(A)[p * L + q];
+    // T = 32 streams 128 B per tile: non-temporal loads keep the tile stream
+    // from displacing x, rowptr and the outputs in L2 (measured faster at
+    // T = 32 only; the smaller tiles lose with them)
+    const uint4 v = T == 32 ? load_nt16(static_cast<const uint4 *>(A) + p * L + q)
+                            : static_cast<const uint4 *>(A)[p * L + q];
     F[0] = v.x;
     F[1] = v.y;
     F[2] = v.z;
@@ -80,7 +91,13 @@ __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *
     const int4 *ci = reinterpret_cast<const int4 *>(colind + pa);
 #pragma unroll
     for (int s = 0; s < S / 4; ++s) {
-      const int4 c = ci[s];
+      int4 c;
+      if constexpr (T == 32) {
+        const uint4 u = load_nt16(ci + s);
+        c = make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
+      } else {
+        c = ci[s];
+      }
       J[4 * s] = c.x;
       J[4 * s + 1] = c.y;
       J[4 * s + 2] = c.z;
@@ -232,6 +249,30 @@ __global__ __launch_bounds__(256) void chunk_rows_kernel(const int *__restrict__
     const int row = __shfl((int)I, src);
     for (long long c = b0 + lane; c < b1; c += 64) part[c] = row;
   }
+}
+
+// Rows [Ia, Ib] of a chunk: Ia holds tile k0 (the chunk's first), Ib holds
+// tile kend (the next chunk's first; nbr-1 for the last chunk). Read from a
+// partition (chunk_rows_kernel), which may be cached from an earlier call on
+// the same rowptr address, so it is checked against rowptr; on a mismatch the
+// rows are searched and the entry is rewritten (vector stores of the same
+// value by every lane). Wave-uniform; must be called by all 64 lanes.
+__device__ __forceinline__ int2 chunk_row_range(const int *__restrict__ rowptr, int *part, int chunk, int nchunks,
+                                                int nbr, long long k0, long long kend, int lane) {
+  const bool last = chunk + 1 >= nchunks;
+  int Ia = part[chunk];
+  int Ib = last ? nbr - 1 : part[chunk + 1];
+  Ia = Ia < 0 ? 0 : (Ia > nbr - 1 ? nbr - 1 : Ia);
+  Ib = Ib < Ia ? Ia : (Ib > nbr - 1 ? nbr - 1 : Ib);
+  bool ok = rowptr[Ia] <= k0 && k0 < rowptr[Ia + 1];
+  if (!last) ok = ok && rowptr[Ib] <= kend && kend < rowptr[Ib + 1];
+  if (!ok) {
+    const int2 ab = wave_find_rows2(rowptr, nbr - 1, k0, last ? k0 : kend, lane);
+    Ia = ab.x;
+    Ib = last ? nbr - 1 : ab.y;
+    part[chunk] = Ia;
+  }
+  return make_int2(Ia, Ib);
 }
 
 // The row ends a wave needs, staged in LDS: rowptr[Ia .. Ib+1], where Ia is

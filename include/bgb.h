@@ -94,6 +94,15 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
 int bgb_set_run_length(int T, int S);
 int bgb_get_run_length(int T);
 
+/* Where the bit-vector BMVs (bin_bin_bin, bin_bin_full, masked) read the
+ * packed x for tile dim T: 0 = automatic (the whole packed x is staged in LDS
+ * by one persistent workgroup per CU when it fits -- about 144 KB of packed
+ * x, i.e. n up to ~1.1M -- and the tile stream is large next to it), 1 =
+ * always gathered from memory, 2 = staged whenever it fits. Process-wide
+ * tuning knob (no reference counterpart). Default 0. */
+int bgb_set_x_staging(int T, int mode);
+int bgb_get_x_staging(int T);
+
 /* C[I] (|)= bits(T-1-r) = OR_c A[I*T+r][c] & x[c].  bmv{T}_bin_bin_bin
  * (bsrbmv.cu:277,437,508,579). B: packed bit vector, C: W_T[NBR]. */
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C,

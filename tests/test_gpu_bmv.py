@@ -210,3 +210,57 @@ def _rmat20():
         from bgb import graph
         _RMAT20 = graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=1)
     return _RMAT20
+
+
+@pytest.mark.parametrize("T", TS)
+def test_rowptr_rewritten_in_place(orc, dev, T):
+    """The chunk partition is cached per (stream, rowptr address, sizes): a caller
+    that rewrites rowptr/colind/val in place with another matrix of the same
+    NBR and nblocks (here: block rows permuted) must still get that matrix's
+    results, on the first call after the rewrite and on the next."""
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(50 + T)
+    csr1 = graph.rmat(13, 8, seed=8)
+    n = csr1.nrows
+    nbr = -(-n // T)
+    perm = rng.permutation(nbr)  # block row I -> perm[I]
+    rows = np.repeat(np.arange(n), np.diff(csr1.rowptr))
+    new_rows = perm[rows // T] * T + rows % T
+    keep = new_rows < n
+    csr2 = graph.from_coo(n, n, new_rows[keep], csr1.colind[keep], False, False)
+    A1 = B.B2SR.from_csr(csr1, T, dev)
+    A2 = B.B2SR.from_csr(csr2, T, dev)
+    if A2.nblocks != A1.nblocks:
+        pytest.skip("permutation changed the tile count (a partial last block row moved)")
+    x01 = (rng.random(n) < 0.3).astype(np.float32)
+    xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+    want1 = orc.csr_count(csr1.rowptr, csr1.colind, x01.astype(np.uint8))
+    want2 = orc.csr_count(csr2.rowptr, csr2.colind, x01.astype(np.uint8))
+    log(f"[case] rowptr rewrite T={T} nblocks={A1.nblocks}")
+    np.testing.assert_array_equal(to_np(B.bmv_bin_bin_full(A1, xb))[:n], want1)
+    A1.rowptr.copy_(A2.rowptr)
+    A1.colind.copy_(A2.colind)
+    A1.val.copy_(A2.val)
+    for _ in range(2):
+        np.testing.assert_array_equal(to_np(B.bmv_bin_bin_full(A1, xb))[:n], want2)
+
+
+@pytest.mark.parametrize("T", TS)
+def test_x_staging_modes(orc, dev, T):
+    """The bit BMVs read the packed x either from memory or from a copy staged
+    in LDS by persistent workgroups (T=4 packs two nibbles per LDS byte): both
+    paths must give the oracle's results, including packed-x lengths that are
+    not a multiple of 16 bytes (the staging copies a tail)."""
+    import bgb
+    from bgb import graph
+    rng = np.random.default_rng(70 + T)
+    try:
+        for mode in (1, 2):
+            bgb.call("bgb_set_x_staging", T, mode)
+            for label, csr in (("rmat14", graph.rmat(14, 8, seed=12)),
+                               ("ragged n=1001", graph.erdos_renyi(1001, 12, seed=3)),
+                               ("ragged n=37", graph.erdos_renyi(37, 6, seed=4))):
+                check_all_ops(orc, csr, T, rng, dev, f"x staging mode {mode} {label}")
+    finally:
+        bgb.call("bgb_set_x_staging", T, 0)

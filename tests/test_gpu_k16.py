@@ -58,9 +58,7 @@ def test_k16_single(orc, dev, T):
         np.testing.assert_allclose(Y2.cpu().numpy(), 2 * Y.cpu().numpy(), rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.skip(reason="the one MI355X run of this case produced no output for 180 s and was stopped; "
-                         "no cause found by reading the code or by a CPU emulation of its indexing (every chunk "
-                         "in bounds, every loop bounded), so the case stays off the GPU until a cause is shown")
+@pytest.mark.timeout(120)
 def test_k16_persistent_batch(orc, dev):
     """Graphs of uneven sizes (log-uniform n), half ER, half R-MAT, mixed T, one launch."""
     from bgb import b2sr as B
@@ -80,6 +78,7 @@ def test_k16_persistent_batch(orc, dev):
         xbs.append(B.k16_pack_rhs(X, A.nblockrows * T))
         refs.append((csr, X64, T))
     batch = B.K16Batch(mats, xbs, nworkgroups=64)
+    log(f"[case] k16 persistent batch: {len(mats)} graphs, 64 workgroups")
     batch.run()
     for i, (csr, X64, T) in enumerate(refs):
         check(orc, csr, batch.result(i), X64, f"batch graph {i} n={csr.nrows} T={T}")

@@ -156,3 +156,20 @@ def test_autotuner_prefers_small_tiles_on_sparse_random_graphs():
     d = graph.from_coo(6400, 6400, src, dst, False, False)
     Td, _ = graph.autotune_tiledim(d, "bin_bin")
     assert Td == 32
+
+
+def test_tuning_knobs_validate_and_round_trip():
+    L = bgb.lib()
+    for T in (4, 8, 16, 32):
+        assert L.bgb_get_run_length(T) in (8, 16)
+        assert L.bgb_get_x_staging(T) == 0
+    assert L.bgb_get_run_length(5) == -1 and L.bgb_get_x_staging(5) == -1
+    assert L.bgb_set_run_length(8, 12) == 1      # BGB_ERR_ARG
+    assert L.bgb_set_x_staging(8, 3) == 1        # BGB_ERR_ARG
+    assert L.bgb_set_x_staging(6, 0) == 2        # BGB_ERR_TILEDIM
+    try:
+        assert L.bgb_set_x_staging(16, 2) == 0 and L.bgb_get_x_staging(16) == 2
+        assert L.bgb_set_run_length(16, 16) == 0 and L.bgb_get_run_length(16) == 16
+    finally:
+        L.bgb_set_x_staging(16, 0)
+        L.bgb_set_run_length(16, 8)
