@@ -149,8 +149,10 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         uint32_t w = row_word<T>(F[s], i) & X[s];
-        if (vr[i] == b.full) w = 0;  // every source already reached this vertex
-        while (w) {
+        // Stop once the partial covers every source that has not reached the
+        // vertex yet: more ORs cannot change acc & ~visited (this includes
+        // vertices every source has reached).
+        while (w && (acc[i] | vr[i]) != b.full) {
           const int h = 31 - __clz(w);
           acc[i] |= fr[T - 1 - h];
           w &= ~(1u << h);

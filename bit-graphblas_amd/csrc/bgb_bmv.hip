@@ -130,30 +130,49 @@ __device__ __forceinline__ uint32_t lds_xword(const uint8_t *xl, int J) {
   else return (uint32_t) reinterpret_cast<const typename Tile<T>::W *>(xl)[J];
 }
 
-// One chunk of CH tiles for one wave (all 64 lanes): `win` is the wave's LDS
-// row window, `xl` the packed x staged in LDS (XL) or unused (x gathered from
-// memory).
+// What one chunk needs before its walk: each lane group's tile run, the
+// raw-clamped row range and the row-window entries in registers. Issued ahead
+// of the walk (a chunk early in the persistent kernel), so that its latency
+// overlaps other work.
+template <int T, int S>
+struct ChunkIn {
+  uint32_t F[S][Tile<T>::NW];
+  int J[S];
+  int wv[kWin / 64];
+  int2 rows;
+};
+
+// Issues the loads of chunk `chunk` with row range `rows` (wave-uniform).
+template <int T, int S>
+__device__ __forceinline__ void load_chunk(const BmvArgs &a, int chunk, int lane, int2 rows, ChunkIn<T, S> &in) {
+  constexpr int L = Tile<T>::L, G = 64 / L, CH = G * S;
+  const int g = lane / L, q = lane % L;
+  const long long k0 = (long long)chunk * CH;
+  const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
+  const long long pb = lmin(pa + S, a.nblocks);
+  if (pa < pb) load_run<T, S>(a.A, a.colind, pa, (int)(pb - pa), q, in.F, in.J);
+  RowWindow::load(a.rowptr, rows.x, rows.y, lane, in.wv);
+  in.rows = rows;
+}
+
+// One chunk of CH tiles for one wave (all 64 lanes) from its loaded inputs:
+// `win` is the wave's LDS row window, `xl` the packed x staged in LDS (XL) or
+// unused (x gathered from memory).
 template <int T, int OP, int S, bool XL>
-__device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane, int *win, const uint8_t *xl) {
+__device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane, int *win, const uint8_t *xl,
+                                          const ChunkIn<T, S> &in) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
-  // rows of this chunk's first tile and of the next chunk's
-  const int2 ab = chunk_row_range(a.rowptr, a.part, chunk, a.nchunks, a.nbr, k0, k0 + CH, lane);
-  const int Ia = ab.x, Ib = ab.y;
   const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
   const long long pb = lmin(pa + S, a.nblocks);
   const bool active = pa < pb;
-  const int *__restrict__ rowptr = a.rowptr;
-
-  // Tile data first: it does not depend on the row search, so its latency
-  // overlaps the search below.
-  uint32_t F[S][NW];
-  int J[S];
   const int n = (int)(pb - pa);
-  if (active) load_run<T, S>(a.A, a.colind, pa, n, q, F, J);
+  const uint32_t(&F)[S][NW] = in.F;
+  const int(&J)[S] = in.J;
+
   uint32_t X[S];
   if constexpr (!is_full(OP)) {
 #pragma unroll
@@ -200,10 +219,10 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   }
 
   // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1], searched in
-  // the wave's staged row ends.
+  // the wave's staged row ends (checked against rowptr first).
   RowWindow rw;
-  rw.stage(rowptr, win, Ia, Ib, lane);
-  int I = active ? rw.find(pa) : Ia;
+  put_checked(rw, a.rowptr, a.part, win, chunk, a.nchunks, a.nbr, k0, k0 + CH, in.rows, lane, in.wv);
+  int I = active ? rw.find(pa) : rw.Ia;
   const int I0 = I;
 
   Acc acc[R], zero[R];
@@ -324,21 +343,42 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
 
 template <int T, int OP, int S>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
+  constexpr int CH = (64 / Tile<T>::L) * S;
   __shared__ int s_win[kWavesPerBlock][kWin];
   const int lane = threadIdx.x & 63;
   const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                    (threadIdx.x >> 6));
   if (chunk >= a.nchunks) return;  // uniform per wave
-  bmv_chunk<T, OP, S, false>(a, chunk, lane, s_win[threadIdx.x >> 6], nullptr);
+  const bool last = chunk + 1 >= a.nchunks;
+  const int2 rows = clamp_rows(a.part[chunk], last ? 0 : a.part[chunk + 1], last, a.nbr);
+  ChunkIn<T, S> in;
+  load_chunk<T, S>(a, chunk, lane, rows, in);
+  (void)CH;
+  bmv_chunk<T, OP, S, false>(a, chunk, lane, s_win[threadIdx.x >> 6], nullptr, in);
+}
+
+// Raw partition entries of chunk c (part[c], part[c+1]) as vector loads, so
+// that they can be in flight across other work (vector loads complete in
+// order, scalar ones would hold up every LDS wait).
+__device__ __forceinline__ int2 load_part_vec(const BmvArgs &a, int c) {
+  int cv = c;
+  asm volatile("" : "+v"(cv));  // a VGPR address: vector loads
+  const int ia = a.part[cv];
+  const int ib = cv + 1 < a.nchunks ? a.part[cv + 1] : 0;
+  return make_int2(ia, ib);
 }
 
 // Bit-vector ops with the whole packed x in LDS (it fits for n up to ~1.1M):
-// one workgroup of kLdsWaves waves per CU (or two, if they fit) stages x
+// one workgroup of lds_waves<T, S>() waves per CU (or two, if they fit) stages x
 // once, then each wave walks chunks gw, gw + W, ... (W waves in the grid).
 // The per-tile x gathers become LDS reads instead of one L2 request each.
-constexpr int kLdsWaves = 16;
+// 16 waves (128 VGPRs each) hold two chunks' inputs for T <= 8 and run
+// length 8; the larger fragments need the 256 VGPRs of 8-wave workgroups.
+template <int T, int S>
+constexpr int lds_waves() { return (T >= 16 || S > 8) ? 8 : 16; }
 template <int T, int OP, int S>
-__global__ __launch_bounds__(64 * kLdsWaves) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
+__global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
+  constexpr int kLdsWaves = lds_waves<T, S>();
   extern __shared__ __align__(16) uint8_t s_dyn[];
   int *win_all = reinterpret_cast<int *>(s_dyn);
   uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWin;
@@ -370,9 +410,35 @@ __global__ __launch_bounds__(64 * kLdsWaves) void bmv_lds_kernel(BmvArgs a, int 
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = gridDim.x * kLdsWaves;
-  for (int c = (int)blockIdx.x * kLdsWaves + wave; c < a.nchunks; c += nw) {
-    const int chunk = __builtin_amdgcn_readfirstlane(c);  // wave-uniform: scalar loads of part / rowptr
-    bmv_chunk<T, OP, S, true>(a, chunk, lane, win_all + wave * kWin, xl);
+  int c = (int)blockIdx.x * kLdsWaves + wave;
+  if (c >= a.nchunks) return;  // uniform per wave; no barrier follows
+  int *win = win_all + wave * kWin;
+  // Software pipeline over this wave's chunks c, c + nw, ...: while chunk c is
+  // walked, the tiles and row window of c + nw and the partition entries of
+  // c + 2nw are in flight.
+  auto rows_of = [&](int cc, int2 raw) {
+    const bool last = cc + 1 >= a.nchunks;
+    return clamp_rows(__builtin_amdgcn_readfirstlane(raw.x), __builtin_amdgcn_readfirstlane(raw.y), last, a.nbr);
+  };
+  // One pipeline step: walk chunk c from `cur` after issuing the loads of
+  // c + nw into `nxt`; returns false after the wave's last chunk. Two buffers
+  // alternate roles, so no chunk inputs are copied.
+  int2 pnext = c + nw < a.nchunks ? load_part_vec(a, c + nw) : make_int2(0, 0);
+  auto step = [&](ChunkIn<T, S> &cur, ChunkIn<T, S> &nxt) {
+    const int chunk = __builtin_amdgcn_readfirstlane(c);
+    const int cn = c + nw;
+    const bool more = cn < a.nchunks;  // wave-uniform
+    if (more) {
+      load_chunk<T, S>(a, cn, lane, rows_of(cn, pnext), nxt);
+      if (cn + nw < a.nchunks) pnext = load_part_vec(a, cn + nw);
+    }
+    bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, cur);
+    c = cn;
+    return more;
+  };
+  ChunkIn<T, S> b0, b1;
+  load_chunk<T, S>(a, c, lane, rows_of(c, load_part_vec(a, c)), b0);
+  while (step(b0, b1) && step(b1, b0)) {
   }
 }
 
@@ -494,11 +560,15 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
     if ((st = device_facts(&f))) return st;
     const long long x_bytes = ceil_div(a.nbr, T) * T * (long long)sizeof(typename Tile<T>::W);
     const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;
+    constexpr int kLdsWaves = lds_waves<T, S>();
     const long long lds = (long long)sizeof(int) * kLdsWaves * kWin + xl;
     const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(typename Tile<T>::W));
     const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
+    // automatic: T <= 16 only (T = 32 streams 132 B per tile and is
+    // bandwidth-bound; the persistent grid's lower occupancy costs more there
+    // than the x gathers it saves, measured on MI355X)
     const int mode = g_x_staging[ilog2_tiledim(T) - 2];
-    const bool worth = mode == 2 || (mode == 0 && tile_bytes >= 2 * xl * f.cus);
+    const bool worth = mode == 2 || (mode == 0 && T <= 16 && tile_bytes >= 2 * xl * f.cus);
     if (mode != 1 && aligned && lds <= f.lds_per_block && worth) {
       static bool attr_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S>),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -79,9 +79,9 @@ __device__ __forceinline__ void load_frag(const void *__restrict__ A, long long 
   }
 }
 
-// Loads the fragments and block columns of tiles [pa, pa+n), n <= S. With
-// n == S and pa a multiple of S (S a multiple of 4) the L=1 tile dims use
-// 16-byte loads over the lane's contiguous run.
+// Loads the fragments and block columns of tiles [pa, pa+n), 1 <= n <= S.
+// With n == S and pa a multiple of S (S a multiple of 4) the L=1 tile dims
+// use 16-byte loads over the lane's contiguous run.
 template <int T, int S>
 __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *__restrict__ colind,
                                          long long pa, int n, int q,
@@ -128,16 +128,13 @@ __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *
       for (int s = 0; s < S; ++s) load_frag<T>(A, pa + s, q, F[s]);
     }
   } else {
+    // a short run (the matrix's last): entries s >= n repeat tile pa+n-1, so
+    // every element is written on both paths (callers only use s < n)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (s < n) {
-        J[s] = colind[pa + s];
-        load_frag<T>(A, pa + s, q, F[s]);
-      } else {
-        J[s] = 0;
-#pragma unroll
-        for (int w = 0; w < Tile<T>::NW; ++w) F[s][w] = 0;
-      }
+      const long long p = pa + (s < n ? s : n - 1);
+      J[s] = colind[p];
+      load_frag<T>(A, p, q, F[s]);
     }
   }
 }
@@ -251,30 +248,6 @@ __global__ __launch_bounds__(256) void chunk_rows_kernel(const int *__restrict__
   }
 }
 
-// Rows [Ia, Ib] of a chunk: Ia holds tile k0 (the chunk's first), Ib holds
-// tile kend (the next chunk's first; nbr-1 for the last chunk). Read from a
-// partition (chunk_rows_kernel), which may be cached from an earlier call on
-// the same rowptr address, so it is checked against rowptr; on a mismatch the
-// rows are searched and the entry is rewritten (vector stores of the same
-// value by every lane). Wave-uniform; must be called by all 64 lanes.
-__device__ __forceinline__ int2 chunk_row_range(const int *__restrict__ rowptr, int *part, int chunk, int nchunks,
-                                                int nbr, long long k0, long long kend, int lane) {
-  const bool last = chunk + 1 >= nchunks;
-  int Ia = part[chunk];
-  int Ib = last ? nbr - 1 : part[chunk + 1];
-  Ia = Ia < 0 ? 0 : (Ia > nbr - 1 ? nbr - 1 : Ia);
-  Ib = Ib < Ia ? Ia : (Ib > nbr - 1 ? nbr - 1 : Ib);
-  bool ok = rowptr[Ia] <= k0 && k0 < rowptr[Ia + 1];
-  if (!last) ok = ok && rowptr[Ib] <= kend && kend < rowptr[Ib + 1];
-  if (!ok) {
-    const int2 ab = wave_find_rows2(rowptr, nbr - 1, k0, last ? k0 : kend, lane);
-    Ia = ab.x;
-    Ib = last ? nbr - 1 : ab.y;
-    part[chunk] = Ia;
-  }
-  return make_int2(Ia, Ib);
-}
-
 // The row ends a wave needs, staged in LDS: rowptr[Ia .. Ib+1], where Ia is
 // the row of the wave's first tile and Ib bounds the row of the first tile
 // after its chunk. When they do not fit (a chunk spanning more than kWin-2
@@ -286,8 +259,23 @@ struct RowWindow {
   int Ia, Ib;
   bool staged;  // wave-uniform
 
+  // Entries rowptr[ia .. ib+1] into registers, kWin/64 per lane (lane + 64j);
+  // issued early by the pipelined kernels. `ia`, `ib` are wave-uniform.
+  __device__ __forceinline__ static void load(const int *__restrict__ rp, int ia, int ib, int lane,
+                                              int (&wv)[kWin / 64]) {
+    const int m = ib + 2 - ia;
+    if (m <= kWin) {
+#pragma unroll
+      for (int j = 0; j < kWin / 64; ++j) {
+        const int i = lane + 64 * j;
+        wv[j] = i < m ? rp[ia + i] : 0;
+      }
+    }
+  }
+  // Makes the window of rows [ia, ib] current, from registers filled by load().
   // Must be called by all 64 lanes.
-  __device__ __forceinline__ void stage(const int *__restrict__ rp, int *w, int ia, int ib, int lane) {
+  __device__ __forceinline__ void put(const int *__restrict__ rp, int *w, int ia, int ib, int lane,
+                                      const int (&wv)[kWin / 64]) {
     rowptr = rp;
     win = w;
     Ia = ia;
@@ -298,13 +286,19 @@ struct RowWindow {
 #pragma unroll
       for (int j = 0; j < kWin / 64; ++j) {
         const int i = lane + 64 * j;
-        if (i < m) win[i] = rowptr[ia + i];
+        if (i < m) win[i] = wv[j];
       }
     }
     // the wave reads entries other lanes wrote (wave-private LDS, in order)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // Must be called by all 64 lanes.
+  __device__ __forceinline__ void stage(const int *__restrict__ rp, int *w, int ia, int ib, int lane) {
+    int wv[kWin / 64];
+    load(rp, ia, ib, lane, wv);
+    put(rp, w, ia, ib, lane, wv);
   }
   // rowptr[i] for Ia <= i <= Ib + 1
   __device__ __forceinline__ int at(int i) const { return staged ? win[i - Ia] : rowptr[i]; }
@@ -319,6 +313,38 @@ struct RowWindow {
     return lo;
   }
 };
+
+// Rows [Ia, Ib] of a chunk: Ia holds tile k0 (the chunk's first), Ib holds
+// tile kend (the next chunk's first; nbr-1 for the last chunk). They come
+// from a partition (chunk_rows_kernel) that may be cached from an earlier
+// call on the same rowptr address, so the raw entries (ia = part[chunk],
+// ib = part[chunk+1]) are clamped here and checked by put_checked.
+__device__ __forceinline__ int2 clamp_rows(int ia, int ib, bool last, int nbr) {
+  int Ia = ia < 0 ? 0 : (ia > nbr - 1 ? nbr - 1 : ia);
+  int Ib = last ? nbr - 1 : (ib < Ia ? Ia : (ib > nbr - 1 ? nbr - 1 : ib));
+  return make_int2(Ia, Ib);
+}
+
+// Puts the window of a chunk's rows from registers (RowWindow::load on the
+// clamped partition entries) and checks them against rowptr: rowptr[Ia] <=
+// k0 < rowptr[Ia+1] and, unless last, rowptr[Ib] <= kend < rowptr[Ib+1]. On
+// a mismatch (rowptr rewritten since the partition was made) the rows are
+// searched, the entry is rewritten (vector stores of the same value by every
+// lane) and the window restaged from memory. Must be called by all lanes.
+__device__ __forceinline__ void put_checked(RowWindow &rw, const int *__restrict__ rowptr, int *part, int *win,
+                                            int chunk, int nchunks, int nbr, long long k0, long long kend,
+                                            int2 rows, int lane, const int (&wv)[kWin / 64]) {
+  const bool last = chunk + 1 >= nchunks;
+  rw.put(rowptr, win, rows.x, rows.y, lane, wv);
+  bool ok = rw.at(rows.x) <= k0 && k0 < rw.at(rows.x + 1);
+  if (!last) ok = ok && rw.at(rows.y) <= kend && kend < rw.at(rows.y + 1);
+  if (!__builtin_amdgcn_readfirstlane((int)ok)) {
+    const int2 ab = wave_find_rows2(rowptr, nbr - 1, k0, last ? k0 : kend, lane);
+    const int Ia = ab.x, Ib = last ? nbr - 1 : ab.y;
+    part[chunk] = Ia;
+    rw.stage(rowptr, win, Ia, Ib, lane);
+  }
+}
 
 // Value of lane (lane - D) inside the lane's 16-lane row (DPP row_shr:D, a
 // VALU operand modifier, no LDS round trip); lanes with (lane % 16) < D get
