@@ -143,14 +143,16 @@ struct ChunkIn {
 };
 
 // Issues the loads of chunk `chunk` with row range `rows` (wave-uniform).
-template <int T, int S>
+// kFull: the chunk's CH tiles all exist (every lane group has a whole run).
+template <int T, int S, bool kFull = false>
 __device__ __forceinline__ void load_chunk(const BmvArgs &a, int chunk, int lane, int2 rows, ChunkIn<T, S> &in) {
   constexpr int L = Tile<T>::L, G = 64 / L, CH = G * S;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin(k0 + (long long)g * S, a.nblocks);
   const long long pb = lmin(pa + S, a.nblocks);
-  if (pa < pb) load_run<T, S>(a.A, a.colind, pa, (int)(pb - pa), q, in.F, in.J);
+  if (kFull) load_run<T, S, true>(a.A, a.colind, pa, S, q, in.F, in.J);
+  else if (pa < pb) load_run<T, S>(a.A, a.colind, pa, (int)(pb - pa), q, in.F, in.J);
   RowWindow::load(a.rowptr, rows.x, rows.y, lane, in.wv);
   in.rows = rows;
 }
@@ -420,25 +422,37 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
     const bool last = cc + 1 >= a.nchunks;
     return clamp_rows(__builtin_amdgcn_readfirstlane(raw.x), __builtin_amdgcn_readfirstlane(raw.y), last, a.nbr);
   };
-  // One pipeline step: walk chunk c from `cur` after issuing the loads of
-  // c + nw into `nxt`; returns false after the wave's last chunk. Two buffers
-  // alternate roles, so no chunk inputs are copied.
-  int2 pnext = c + nw < a.nchunks ? load_part_vec(a, c + nw) : make_int2(0, 0);
+  // One pipeline step over whole chunks (c < nfull, every run has S tiles):
+  // walk chunk c from `cur` after issuing the loads of c + nw into `nxt`;
+  // returns false after the wave's last whole chunk. Two buffers alternate
+  // roles, so no chunk inputs are copied.
+  constexpr int CH = (64 / Tile<T>::L) * S;
+  const int nfull = a.nblocks / CH;  // the last chunk may be short
+  int2 pnext = c + nw < nfull ? load_part_vec(a, c + nw) : make_int2(0, 0);
   auto step = [&](ChunkIn<T, S> &cur, ChunkIn<T, S> &nxt) {
     const int chunk = __builtin_amdgcn_readfirstlane(c);
     const int cn = c + nw;
-    const bool more = cn < a.nchunks;  // wave-uniform
+    const bool more = cn < nfull;  // wave-uniform
     if (more) {
-      load_chunk<T, S>(a, cn, lane, rows_of(cn, pnext), nxt);
-      if (cn + nw < a.nchunks) pnext = load_part_vec(a, cn + nw);
+      load_chunk<T, S, true>(a, cn, lane, rows_of(cn, pnext), nxt);
+      if (cn + nw < nfull) pnext = load_part_vec(a, cn + nw);
     }
     bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, cur);
     c = cn;
     return more;
   };
-  ChunkIn<T, S> b0, b1;
-  load_chunk<T, S>(a, c, lane, rows_of(c, load_part_vec(a, c)), b0);
-  while (step(b0, b1) && step(b1, b0)) {
+  if (c < nfull) {
+    ChunkIn<T, S> b0, b1;
+    load_chunk<T, S, true>(a, c, lane, rows_of(c, load_part_vec(a, c)), b0);
+    while (step(b0, b1) && step(b1, b0)) {
+    }
+  }
+  // the short last chunk, if any, by the wave whose sequence reaches it
+  if (c == nfull && nfull < a.nchunks) {
+    const int chunk = __builtin_amdgcn_readfirstlane(c);
+    ChunkIn<T, S> in;
+    load_chunk<T, S>(a, chunk, lane, rows_of(chunk, load_part_vec(a, chunk)), in);
+    bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, in);
   }
 }
 

@@ -82,12 +82,14 @@ __device__ __forceinline__ void load_frag(const void *__restrict__ A, long long 
 // Loads the fragments and block columns of tiles [pa, pa+n), 1 <= n <= S.
 // With n == S and pa a multiple of S (S a multiple of 4) the L=1 tile dims
 // use 16-byte loads over the lane's contiguous run.
-template <int T, int S>
+// kFull: the caller guarantees n == S (no short-run code is emitted, which
+// keeps the compiler from merging the two paths into per-dword loads).
+template <int T, int S, bool kFull = false>
 __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *__restrict__ colind,
                                          long long pa, int n, int q,
                                          uint32_t (&F)[S][Tile<T>::NW], int (&J)[S]) {
   static_assert(S % 4 == 0, "run length must be a multiple of 4");
-  if (n == S) {
+  if (kFull || n == S) {
     const int4 *ci = reinterpret_cast<const int4 *>(colind + pa);
 #pragma unroll
     for (int s = 0; s < S / 4; ++s) {
@@ -127,14 +129,31 @@ __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *
 #pragma unroll
       for (int s = 0; s < S; ++s) load_frag<T>(A, pa + s, q, F[s]);
     }
-  } else {
-    // a short run (the matrix's last): entries s >= n repeat tile pa+n-1, so
-    // every element is written on both paths (callers only use s < n)
+  } else if constexpr (kFull) {
+  } else if constexpr (T <= 8) {
+    // A short run (the matrix's last). Entries s >= n repeat tile pa+n-1, so
+    // every element is written unconditionally (callers only use s < n);
+    // conditional writes here leave a pointer select over two stack slots
+    // (scratch) in the T = 4 kernels.
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const long long p = pa + (s < n ? s : n - 1);
       J[s] = colind[p];
       load_frag<T>(A, p, q, F[s]);
+    }
+  } else {
+    // T >= 16: zero-filled; the unconditional form above would be merged
+    // with the full-run path (the same load_frag calls) into per-dword loads.
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (s < n) {
+        J[s] = colind[pa + s];
+        load_frag<T>(A, pa + s, q, F[s]);
+      } else {
+        J[s] = 0;
+#pragma unroll
+        for (int w = 0; w < Tile<T>::NW; ++w) F[s][w] = 0;
+      }
     }
   }
 }
