@@ -41,6 +41,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "bmv_traffic.json")
+
+
+def measured_traffic(workload, T):
+    """HBM bytes per BMV call at tile dim T for `workload`, from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --profile`
+    (main kernel + fix-up, each counter scaled by what it reads for a 256 MiB
+    copy in the same run), or None when not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(workload, {}).get(str(T))
+    return e["bytes_per_call"] if e else None
+
+
 def bmv_bytes(T, nbr, nblocks):
     wb = {4: 1, 8: 1, 16: 2, 32: 4}[T]
     x = -(-nbr // T) * T * wb
@@ -219,11 +236,16 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
            "bin_full_full_per_tile": per_tile}
     ref = None
     if with_cpu:
+        # CPU baseline of this section: the oracle's FP64 power iteration
+        # (OpenMP), timed; the GPU modes' errors are stated against its result
         import oracle_lib
         orc = oracle_lib.Oracle()
         t0 = time.perf_counter()
         ref, _ = orc.pagerank(csr.rowptr, csr.colind, 0.85, iters)
-        out["cpu_fp64_s"] = round(time.perf_counter() - t0, 2)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(tc * 1e3, 1), "unit": f"ms per {iters} iterations",
+                               "cores": orc.threads(), "kind": "port",
+                               "sample": f"oracle FP64 PageRank, the full {side}^2 grid, {iters} iterations"}
     modes = {}
     for mode in ("fp32", "bf16", "fp64sum"):
         B.pagerank(A, 0.85, 2, mode)
@@ -252,7 +274,11 @@ def main():
     ap.add_argument("--config", default="rmat20", choices=sorted(CONFIGS))
     ap.add_argument("--tiles", default="4,8,16,32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--extras", default="", help="comma list: bfs, rmat22, pagerank")
+    ap.add_argument("--extras", default="rmat22,bfs,pagerank",
+                    help="comma list of sections beside the R-MAT 20 sweep: rmat22 (bin_bin_full on R-MAT "
+                         "scale 22), bfs (batched 64-source BFS vs the plain per-source loop on R-MAT 22), "
+                         "pagerank (4096^2 grid bin_full_full + PageRank in three precisions); 'none' for "
+                         "the sweep only")
     ap.add_argument("--bfs-tile", type=int, default=0, help="tile dim for the BFS section (0: autotune)")
     ap.add_argument("--run-length", type=int, default=0,
                     help="tiles per lane group in the BMV kernels (8 or 16; 0 keeps the library default)")
@@ -353,7 +379,7 @@ def main():
         log(f"[bench] cpu {cpu['value']:.3f} GTEPS on {cpu['cores']} threads, match={match}")
 
     extras = {}
-    wanted = [w for w in args.extras.split(",") if w]
+    wanted = [w for w in args.extras.split(",") if w and w != "none"]
     if "rmat22" in wanted or "bfs" in wanted:
         log("[bench] generating R-MAT scale 22 ef 16")
         t0 = time.perf_counter()
@@ -403,7 +429,9 @@ def main():
             "run_length": run_lengths[best], "x_staging": x_staging[best],
         },
         "roofline": {"bound": "hbm", "achieved": round(b["alg_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(b["hbm_frac"], 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(b["hbm_frac"], 4),
+                     "traffic": measured_traffic(args.config, best),
+                     "alg_bytes": b["alg_bytes"]},
         "cpu_baseline": cpu,
         "copy_ceiling_gbs": round(ceiling, 1),
         "read_ceiling_gbs": round(rceil, 1) if rceil else None,

@@ -20,24 +20,38 @@ namespace {
 const int kTs[4] = {4, 8, 16, 32};
 
 // Bytes and issue constants per T (index 0..3 = T 4/8/16/32), per op family.
-// tile_ns: wave-issue time per tile at full occupancy (ns, whole chip);
+// tile_ns: whole-chip time per tile of the chunked kernels (ns);
 // bw_gbs: sustained HBM bandwidth of the streaming loop.
 struct Model {
   double tile_ns[4];
   double row_ns[4];
   double bw_gbs;
 };
-// op 0: bin_bin_full / bin_bin_bin, op 1: bin_full_full (FP32), op 2: K=16 MFMA,
-// op 3: one level of the batched 64-source BFS
-// Fitted to MI355X runs of the chunked kernels on R-MAT scale 20 / 22 and the
-// 4096^2 grid (whole-chip time per tile at full occupancy; the small tiles are
-// issue/latency bound, T = 32 is bandwidth bound at ~4.2 TB/s).
+// op 0: bin_bin_full / bin_bin_bin with x gathered from memory, op 1:
+// bin_full_full (FP32), op 2: K=16 MFMA, op 3: one level of the batched
+// 64-source BFS. Fitted to MI355X runs on R-MAT scale 20 / 22 (bit ops) and the
+// 4096^2 grid (FP32): T <= 16 is bound by per-tile issue and latency, T = 32 by
+// HBM at ~5.3 TB/s.
 const Model kModels[4] = {
-    {{0.0063, 0.0063, 0.0118, 0.0200}, {0.0020, 0.0030, 0.0050, 0.0100}, 4200.0},
-    {{0.0250, 0.0300, 0.0600, 0.1500}, {0.0030, 0.0050, 0.0100, 0.0200}, 4000.0},
+    {{0.0046, 0.0057, 0.0102, 0.0200}, {0.0010, 0.0010, 0.0010, 0.0010}, 5300.0},
+    {{0.0055, 0.0095, 0.0450, 0.0580}, {0.0010, 0.0010, 0.0010, 0.0010}, 4000.0},
     {{1e9, 1e9, 0.0600, 0.2000}, {1e9, 1e9, 0.0300, 0.0600}, 4000.0},
-    {{0.0132, 0.0191, 0.0350, 0.0800}, {0.0050, 0.0080, 0.0150, 0.0300}, 4000.0},
+    {{0.0117, 0.0160, 0.0300, 0.0700}, {0.0050, 0.0080, 0.0150, 0.0300}, 4000.0},
 };
+// op 0 when the packed x is staged in LDS (persistent workgroups, T <= 16):
+// per-tile costs measured on R-MAT scale 20.
+const double kTileNsLds[4] = {0.0036, 0.0044, 0.0091, 0.0200};
+// LDS a workgroup may use on gfx950, minus the per-wave row windows (16 x 1 KB).
+constexpr long long kLdsForX = 160 * 1024 - 16 * 1024;
+
+// Packed-x bytes the staging kernel keeps in LDS for n rows at tile dim T
+// (T = 4: two nibbles per byte).
+long long lds_x_bytes(int T, long long nrows) {
+  const long long nbr = (nrows + T - 1) / T;
+  const long long wb = T == 32 ? 4 : T == 16 ? 2 : 1;
+  const long long x = (nbr + T - 1) / T * T * wb;
+  return T == 4 ? (x + 1) / 2 : x;
+}
 
 double op_bytes(int op, int T, long long nrows, long long nblocks) {
   const double wb = T == 32 ? 4 : T == 16 ? 2 : 1;
@@ -95,7 +109,9 @@ int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int all
     const int T = kTs[t];
     const double nbr = (double)((nrows + T - 1) / T);
     const double mem_us = op_bytes(op, T, nrows, h_nblocks4[t]) / (m.bw_gbs * 1e3);
-    const double issue_us = ((double)h_nblocks4[t] * m.tile_ns[t] + nbr * m.row_ns[t]) / 1e3;
+    const bool lds = op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX;
+    const double tile_ns = lds ? kTileNsLds[t] : m.tile_ns[t];
+    const double issue_us = ((double)h_nblocks4[t] * tile_ns + nbr * m.row_ns[t]) / 1e3;
     const double c = std::max(mem_us, issue_us) + 3.0;  // + launch overhead
     if (cost4_us) cost4_us[t] = c;
     if ((allowed_mask >> t) & 1) {
