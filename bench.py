@@ -266,6 +266,147 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
     return out
 
 
+def k16_section(torch, B, graph, csr, with_cpu=True, steps=20):
+    """bin_full_full with K=16 FP16 right-hand sides on MFMA (FP32 accumulate) on one graph,
+    T = 16 and 32; error against the oracle's FP64 SpMM of the same FP16 values, relative to
+    sum |A||X| per entry (the FP32-accumulation bound)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(16)
+    X = (torch.rand((csr.nrows, 16), generator=g, device=dev) * 2 - 1).to(torch.float16)
+    out = {"n": csr.nrows, "nnz": csr.nnz, "rhs": 16, "per_tile": {}}
+    want = bound = None
+    if with_cpu:
+        import oracle_lib
+        orc = oracle_lib.Oracle()
+        X64 = X.cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        want = orc.csr_spmm_f64(csr.rowptr, csr.colind, X64)
+        tc = time.perf_counter() - t0
+        bound = orc.csr_spmm_f64(csr.rowptr, csr.colind, np.abs(X64))
+        out["cpu_baseline"] = {"value": round(csr.nnz / tc / 1e9, 3), "unit": "GTEPS (16 RHS)",
+                               "cores": orc.threads(), "kind": "port",
+                               "sample": "oracle FP64 CSR SpMM (OpenMP), the full graph, one pass"}
+    for T in (16, 32):
+        A = B.B2SR.from_csr(csr, T, dev)
+        Xb = B.k16_pack_rhs(X, A.nblockrows * T)
+        Y = torch.empty((A.nblockrows * T, 16), dtype=torch.float32, device=dev)
+        t, _ = time_bmv(torch, lambda: B.bmv_k16(A, Xb, out=Y), steps, 3)
+        wb = {16: 2, 32: 4}[T]
+        nbytes = (4 * (A.nblockrows + 1) + A.nblocks * (4 + T * wb) + Xb.numel() * 2
+                  + 4 * 16 * A.nblockrows * T)
+        m = {"ms": round(t * 1e3, 4), "gteps": round(csr.nnz / t / 1e9, 2), "nblocks": A.nblocks,
+             "alg_gbs": round(nbytes / t / 1e9, 1), "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+             "tflops": round(2.0 * 16 * A.nblocks * T * T / t / 1e12, 2)}
+        if want is not None:
+            got = Y[: csr.nrows].cpu().numpy().astype(np.float64)
+            m["max_err_rel_to_bound"] = float(np.max(np.abs(got - want) / np.maximum(bound, 1e-30)))
+        out["per_tile"][str(T)] = m
+        log(f"[bench] k16 T={T}: {m}")
+        del A, Xb, Y
+        torch.cuda.empty_cache()
+    return out
+
+
+def k16batch_section(torch, B, graph, ngraphs=1000, nmin=1e3, nmax=1e6, batch_bytes=12 << 30, with_cpu=True,
+                     seed=5):
+    """bin_full_full with K=16 FP16 right-hand sides (uniform in [-1, 1]) on MFMA, FP32
+    accumulate, over `ngraphs` graphs of log-uniform n in [nmin, nmax] -- half
+    Erdos-Renyi with average degree U{8..32}, half R-MAT with edge factor 8 (scale
+    round(log2 n), so its n is the nearest power of two) -- tile dim autotuned per graph.
+    Graphs stream through HBM in batches of ~batch_bytes; each batch is one persistent
+    launch (bgb_k16_batched) timed with HIP events. Every graph's result is compared
+    with the oracle's FP64 SpMM of the same FP16 values: error = max |Y - Y64| / (sum_u
+    |A_vu| |X_u|) over entries (the FP32-accumulation bound)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    gen_t = torch.Generator(device=dev)
+    gen_t.manual_seed(seed)
+    orc = None
+    if with_cpu:
+        import oracle_lib
+        orc = oracle_lib.Oracle()
+    specs = []
+    for i in range(ngraphs):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        specs.append(("er", n, int(rng.integers(8, 33))) if i % 2 == 0 else ("rmat", n, 8))
+    tot = {"graphs": 0, "n": 0, "nnz": 0, "batches": 0, "gpu_s": 0.0, "cpu_s": 0.0, "gen_s": 0.0}
+    tdims = {16: 0, 32: 0}
+    errs = []
+    pending = []  # (csr, A, X, Xb)
+    pending_bytes = 0
+
+    def flush():
+        nonlocal pending, pending_bytes
+        if not pending:
+            return
+        batch = B.K16Batch([p[1] for p in pending], [p[3] for p in pending], nworkgroups=0)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        batch.run()
+        e.record()
+        torch.cuda.synchronize()
+        tot["gpu_s"] += s.elapsed_time(e) / 1e3
+        tot["batches"] += 1
+        log(f"[bench] k16 batch {tot['batches']}: {len(pending)} graphs, {s.elapsed_time(e):.2f} ms")
+        for i, (csr, A, X, Xb) in enumerate(pending):
+            if orc is None:
+                continue
+            X64 = X.cpu().numpy().astype(np.float64)
+            t0 = time.perf_counter()
+            want = orc.csr_spmm_f64(csr.rowptr, csr.colind, X64)
+            tot["cpu_s"] += time.perf_counter() - t0
+            bound = orc.csr_spmm_f64(csr.rowptr, csr.colind, np.abs(X64))
+            got = batch.result(i)[: csr.nrows].cpu().numpy().astype(np.float64)
+            errs.append(float(np.max(np.abs(got - want) / np.maximum(bound, 1e-30))))
+        del batch
+        pending, pending_bytes = [], 0
+        torch.cuda.empty_cache()
+
+    for i, (kind, n, deg) in enumerate(specs):
+        t0 = time.perf_counter()
+        if kind == "er":
+            csr = graph.erdos_renyi(n, deg, seed=1000 + i)
+        else:
+            csr = graph.rmat(max(10, int(round(np.log2(n)))), deg, seed=1000 + i)
+        tot["gen_s"] += time.perf_counter() - t0
+        T, _ = graph.autotune_tiledim(csr, "k16", allowed=(16, 32))
+        A = B.B2SR.from_csr(csr, T, dev)
+        X = (torch.rand((csr.nrows, 16), generator=gen_t, device=dev) * 2 - 1).to(torch.float16)
+        Xb = B.k16_pack_rhs(X, A.nblockrows * T)
+        pending.append((csr, A, X, Xb))
+        pending_bytes += (A.val.numel() * A.val.element_size() + 4 * A.nblocks + Xb.numel() * 2
+                          + 4 * 16 * A.nblockrows * T)
+        tdims[T] += 1
+        tot["graphs"] += 1
+        tot["n"] += csr.nrows
+        tot["nnz"] += csr.nnz
+        if (i + 1) % 50 == 0:
+            log(f"[bench] k16 batch: {i + 1} graphs generated, packed ({tot['gen_s']:.0f} s generating)")
+        if pending_bytes >= batch_bytes:
+            flush()
+    flush()
+    out = {"graphs": tot["graphs"], "total_n": tot["n"], "total_nnz": tot["nnz"], "batches": tot["batches"],
+           "tile_dims": {str(k): v for k, v in tdims.items()},
+           "gpu_ms": round(tot["gpu_s"] * 1e3, 2), "gteps": round(tot["nnz"] / tot["gpu_s"] / 1e9, 2),
+           "rhs": 16, "gen_s": round(tot["gen_s"], 1)}
+    if errs:
+        e = np.array(errs)
+        out["error_vs_fp64"] = {"bound": "max |Y - Y64| / (sum |A||X|) per graph", "max": float(e.max()),
+                                "median": float(np.median(e)), "within_1e-6": int((e <= 1e-6).sum()),
+                                "per_graph": [float(f"{v:.3g}") for v in errs]}
+        out["cpu_baseline"] = {"value": round(tot["nnz"] / tot["cpu_s"] / 1e9, 3), "unit": "GTEPS (16 RHS)",
+                               "cores": orc.threads(), "kind": "port",
+                               "sample": "oracle FP64 CSR SpMM (OpenMP) over every graph of the batch"}
+    log(f"[bench] k16 batch: {{k: v for k, v in out.items() if k != 'error_vs_fp64'}}")
+    if errs:
+        log(f"[bench] k16 batch error vs FP64: max {max(errs):.3e}, within 1e-6: {out['error_vs_fp64']['within_1e-6']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,12 +415,15 @@ def main():
     ap.add_argument("--config", default="rmat20", choices=sorted(CONFIGS))
     ap.add_argument("--tiles", default="4,8,16,32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--extras", default="rmat22,bfs,pagerank",
+    ap.add_argument("--extras", default="rmat22,bfs,pagerank,k16",
                     help="comma list of sections beside the R-MAT 20 sweep: rmat22 (bin_bin_full on R-MAT "
                          "scale 22), bfs (batched 64-source BFS vs the plain per-source loop on R-MAT 22), "
-                         "pagerank (4096^2 grid bin_full_full + PageRank in three precisions); 'none' for "
-                         "the sweep only")
+                         "pagerank (4096^2 grid bin_full_full + PageRank in three precisions), k16 (K=16 MFMA "
+                         "bin_full_full on R-MAT 20), k16batch "
+                         "(K=16 MFMA bin_full_full over 1000 graphs of uneven size, persistent launches); "
+                         "'none' for the sweep only")
     ap.add_argument("--bfs-tile", type=int, default=0, help="tile dim for the BFS section (0: autotune)")
+    ap.add_argument("--k16-graphs", type=int, default=1000, help="graphs in the k16batch section")
     ap.add_argument("--run-length", type=int, default=0,
                     help="tiles per lane group in the BMV kernels (8 or 16; 0 keeps the library default)")
     ap.add_argument("--x-staging", type=int, default=-1,
@@ -407,6 +551,11 @@ def main():
         del csr22
     if "pagerank" in wanted:
         extras["pagerank_grid4096"] = pagerank_section(torch, B, graph, with_cpu=not args.no_cpu)
+    if "k16" in wanted:
+        extras["k16_rmat20"] = k16_section(torch, B, graph, csr, with_cpu=not args.no_cpu)
+    if "k16batch" in wanted:
+        extras["k16_persistent_batch"] = k16batch_section(torch, B, graph, ngraphs=args.k16_graphs,
+                                                          with_cpu=not args.no_cpu)
 
     b = per_tile[best]
     line = {
