@@ -401,7 +401,7 @@ def k16batch_section(torch, B, graph, ngraphs=1000, nmin=1e3, nmax=1e6, batch_by
         out["cpu_baseline"] = {"value": round(tot["nnz"] / tot["cpu_s"] / 1e9, 3), "unit": "GTEPS (16 RHS)",
                                "cores": orc.threads(), "kind": "port",
                                "sample": "oracle FP64 CSR SpMM (OpenMP) over every graph of the batch"}
-    log(f"[bench] k16 batch: {{k: v for k, v in out.items() if k != 'error_vs_fp64'}}")
+    log("[bench] k16 batch: " + str({k: v for k, v in out.items() if k != "error_vs_fp64"}))
     if errs:
         log(f"[bench] k16 batch error vs FP64: max {max(errs):.3e}, within 1e-6: {out['error_vs_fp64']['within_1e-6']}")
     return out

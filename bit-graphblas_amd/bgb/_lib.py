@@ -59,7 +59,9 @@ SIGNATURES = {
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_autotune_tiledim": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp]),
+    "bgb_autotune": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_csr_from_coo": (_c_int, [_c_int, _c_int, _c_ll, _vp, _vp, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "bgb_read_mtx": (_c_int, [ctypes.c_char_p, _c_int, ctypes.POINTER(_vp)]),
     "bgb_gen_er": (_c_int, [_c_int, _c_ll, _c_u64, ctypes.POINTER(_vp)]),
     "bgb_gen_rmat": (_c_int, [_c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_u64, ctypes.POINTER(_vp)]),
     "bgb_gen_grid2d": (_c_int, [_c_int, _c_int, ctypes.POINTER(_vp)]),

@@ -1,5 +1,6 @@
 """Host-side CSR graphs from libbgb's generators (numpy views of int32 CSR)."""
 import ctypes
+import os
 
 import numpy as np
 
@@ -55,6 +56,13 @@ def from_coo(nrows, ncols, src, dst, undirected=True, drop_self_loops=True):
                 int(undirected), int(drop_self_loops))
 
 
+def read_mtx(path, directed=0):
+    """Normalised CSR from a MatrixMarket coordinate file (the reference's readMtx
+    rules, readMtx.hpp:353-422): directed 0 = undirected iff the banner says
+    symmetric, 1 = directed, 2 = undirected."""
+    return _gen("bgb_read_mtx", os.fsencode(path), int(directed))
+
+
 def erdos_renyi(n, avg_degree, seed):
     """G(n, m) with m = n*avg_degree/2 distinct undirected edges, no self-loops."""
     return _gen("bgb_gen_er", n, int(n * avg_degree) // 2, seed)
@@ -90,6 +98,19 @@ def tile_stats(csr):
 
 
 OP_FAMILY = {"bin_bin": 0, "bin_full_full": 1, "k16": 2, "bfs": 3}
+
+
+def autotune(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):
+    """Per-graph kernel configuration: {"T", "run_length", "x_staging", "cost_us"}."""
+    nbd, _ = stats if stats is not None else tile_stats(csr)
+    nb = np.array([nbd[T] for T in (4, 8, 16, 32)], np.int64)
+    cost = np.zeros(4, np.float64)
+    mask = sum(1 << i for i, T in enumerate((4, 8, 16, 32)) if T in allowed)
+    t, rl, xs = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    call("bgb_autotune", OP_FAMILY[op], csr.nrows, nb.ctypes.data, mask, ctypes.byref(t), ctypes.byref(rl),
+         ctypes.byref(xs), cost.ctypes.data)
+    return {"T": t.value, "run_length": rl.value, "x_staging": xs.value,
+            "cost_us": {T: float(cost[i]) for i, T in enumerate((4, 8, 16, 32))}}
 
 
 def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):

@@ -246,8 +246,8 @@ __global__ void bfs_advance_kernel(int *ctl) {
 
 template <class V>
 __global__ void bfs_fill_kernel(V *p, long long n, V v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // Sources: V[src] and F[src] get bit s; the packed frontier word gets the bit.
@@ -311,6 +311,8 @@ __global__ void plain_seed_kernel(int src, void *front, void *visited, int *leve
 }
 
 unsigned nblk(long long n, int b = 256) { return (unsigned)ceil_div(n > 0 ? n : 1, b); }
+// grids of the grid-stride fills (up to 16 words per vertex)
+unsigned nfill(long long n) { return capped_blocks(n, 256); }
 
 template <int T>
 int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
@@ -335,15 +337,15 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   if (nchunks > 0)  // chunk -> first block row, once for every level
     chunk_rows_kernel<<<nblk(nbr), 256, 0, s>>>(rowptr, nbr, CH, (int)nchunks, pt.as<int>());
   const uint64_t full = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
-  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F0.as<uint64_t>(), npad, 0ull);
-  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(F1.as<uint64_t>(), npad, 0ull);
-  bfs_fill_kernel<uint64_t><<<nblk(npad * kPlanes), 256, 0, s>>>(Pb.as<uint64_t>(), npad * kPlanes, 0ull);
+  bfs_fill_kernel<uint64_t><<<nfill(npad), 256, 0, s>>>(F0.as<uint64_t>(), npad, 0ull);
+  bfs_fill_kernel<uint64_t><<<nfill(npad), 256, 0, s>>>(F1.as<uint64_t>(), npad, 0ull);
+  bfs_fill_kernel<uint64_t><<<nfill(npad * kPlanes), 256, 0, s>>>(Pb.as<uint64_t>(), npad * kPlanes, 0ull);
   // padding rows count as visited by every source
-  bfs_fill_kernel<uint64_t><<<nblk(npad), 256, 0, s>>>(Vb.as<uint64_t>(), npad, 0ull);
+  bfs_fill_kernel<uint64_t><<<nfill(npad), 256, 0, s>>>(Vb.as<uint64_t>(), npad, 0ull);
   if (npad > n)
-    bfs_fill_kernel<uint64_t><<<nblk(npad - n), 256, 0, s>>>(Vb.as<uint64_t>() + n, npad - n, full);
-  bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(z0.as<uint8_t>(), wb * nwords + 16, 0);
-  bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(z1.as<uint8_t>(), wb * nwords + 16, 0);
+    bfs_fill_kernel<uint64_t><<<nfill(npad - n), 256, 0, s>>>(Vb.as<uint64_t>() + n, npad - n, full);
+  bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z0.as<uint8_t>(), wb * nwords + 16, 0);
+  bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z1.as<uint8_t>(), wb * nwords + 16, 0);
   const int init_ctl[4] = {1, 0, 0, 0};
   BGB_HIP_TRY(hipMemcpyAsync(ctl.ptr, init_ctl, sizeof(init_ctl), hipMemcpyHostToDevice, s));
   BGB_HIP_TRY(hipMemcpyAsync(src.ptr, h_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, s));
@@ -465,8 +467,8 @@ int bfs_plain_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   BGB_HIP_TRY(vis.alloc(wb * nwords + 16, s));
   BGB_HIP_TRY(any.alloc(sizeof(int), s));
   for (DeviceBuffer *d : {&f0, &f1, &vis})
-    bfs_fill_kernel<uint8_t><<<nblk(wb * nwords + 16), 256, 0, s>>>(d->as<uint8_t>(), wb * nwords + 16, 0);
-  bfs_fill_kernel<int><<<nblk(n), 256, 0, s>>>(level, n, -1);
+    bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(d->as<uint8_t>(), wb * nwords + 16, 0);
+  bfs_fill_kernel<int><<<nfill(n), 256, 0, s>>>(level, n, -1);
   plain_seed_kernel<T><<<1, 64, 0, s>>>(source, f0.ptr, vis.ptr, level);
   BGB_HIP_TRY(hipGetLastError());
   void *cur = f0.ptr, *nxt = f1.ptr;

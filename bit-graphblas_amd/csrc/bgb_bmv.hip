@@ -547,8 +547,8 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
 
 template <class V>
 __global__ void fill_kernel(V *p, long long n, V v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 template <int T, int OP, int S>
@@ -638,7 +638,7 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
     if (accumulate && OP != OP_MASKED) return BGB_OK;
     const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)nbr * word_bytes(T)
                                                            : (long long)nbr * T * 4;
-    fill_kernel<uint8_t><<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(static_cast<uint8_t *>(C), n, 0);
+    fill_kernel<uint8_t><<<capped_blocks(n, 256), 256, 0, s>>>(static_cast<uint8_t *>(C), n, 0);
     BGB_HIP_TRY(hipGetLastError());
     return BGB_OK;
   }

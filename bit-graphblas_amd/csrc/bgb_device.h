@@ -52,6 +52,16 @@ struct DeviceFacts {
 };
 int device_facts(DeviceFacts *out);
 
+// Workgroups for n items of `block` threads, capped: the dispatch packet
+// carries the grid size in work-items as a 32-bit number, so launches whose
+// item count can pass 2^32 (per-carry-value fix-ups, fills of 16 words per
+// vertex) use at most 2^20 workgroups and walk their range with a
+// grid-stride loop.
+inline unsigned capped_blocks(long long n, int block) {
+  const long long b = (n + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : (b > (1LL << 20) ? (1LL << 20) : b));
+}
+
 inline int ilog2_tiledim(int T) { return T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : 5; }
 
 }  // namespace bgb

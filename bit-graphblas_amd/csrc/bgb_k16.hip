@@ -160,20 +160,22 @@ __global__ __launch_bounds__(64 * kWaves) void k16_kernel(K16Args a) {
 __global__ void k16_fixup_kernel(const int *carry_key, const float *carry_val, long long nchunks,
                                  const K16Graph *graphs, const int *graph_of, const int *tdim, K16Graph single,
                                  int single_T) {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nchunks * 32 * kK) return;
-  const long long c = tid / (32 * kK);
-  const int e = (int)(tid % (32 * kK));
-  const int key = carry_key[c];
-  if (key < 0) return;
-  const int gi = graph_of ? graph_of[c] : 0;
-  const int T = graph_of ? tdim[gi] : single_T;
-  const int row = e / kK, col = e % kK;
-  if (row >= T) return;
-  const float v = carry_val[tid];
-  if (v == 0.0f) return;
-  float *y = graph_of ? graphs[gi].y : single.y;
-  unsafeAtomicAdd(y + ((long long)key * T + row) * kK + col, v);
+  const long long total = nchunks * 32 * kK;
+  for (long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x; tid < total;
+       tid += (long long)gridDim.x * blockDim.x) {
+    const long long c = tid / (32 * kK);
+    const int e = (int)(tid % (32 * kK));
+    const int key = carry_key[c];
+    if (key < 0) continue;
+    const int gi = graph_of ? graph_of[c] : 0;
+    const int T = graph_of ? tdim[gi] : single_T;
+    const int row = e / kK, col = e % kK;
+    if (row >= T) continue;
+    const float v = carry_val[tid];
+    if (v == 0.0f) continue;
+    float *y = graph_of ? graphs[gi].y : single.y;
+    unsafeAtomicAdd(y + ((long long)key * T + row) * kK + col, v);
+  }
 }
 
 // Persistent launch over many graphs: waves pull chunk indices from a queue.
@@ -227,21 +229,23 @@ __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArg
 
 // Xb[(b*64 + l)*4 + j] = X[(16b + 4(l/16) + j)*16 + l%16], zero beyond n.
 __global__ void k16_pack_rhs_kernel(const uint16_t *X, long long n, long long nblk16, uint16_t *Xb) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblk16 * 256) return;
-  const long long b = t / 256;
-  const int r = (int)(t % 256), l = r / 4, j = r % 4;
-  const long long v = 16 * b + 4 * (l / 16) + j;
-  Xb[t] = v < n ? X[v * kK + (l % 16)] : (uint16_t)0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nblk16 * 256;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long b = t / 256;
+    const int r = (int)(t % 256), l = r / 4, j = r % 4;
+    const long long v = 16 * b + 4 * (l / 16) + j;
+    Xb[t] = v < n ? X[v * kK + (l % 16)] : (uint16_t)0;
+  }
 }
 
 template <class V>
 __global__ void k16_fill_kernel(V *p, long long n, V v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
-unsigned nb256(long long n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
+// grids of the grid-stride kernels above
+unsigned nb256(long long n) { return capped_blocks(n, 256); }
 
 int k16_check(int T, const void *A, const void *Xb, const float *Y, const int *rowptr, const int *colind, int nbr,
               int nblocks) {

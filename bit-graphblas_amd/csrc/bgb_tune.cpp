@@ -97,6 +97,18 @@ int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colin
   return BGB_OK;
 }
 
+int bgb_autotune(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
+                 int *run_length_out, int *x_staging_out, double *cost4_us) {
+  int T = 0;
+  const int st = bgb_autotune_tiledim(op, nrows, h_nblocks4, allowed_mask, &T, cost4_us);
+  if (st) return st;
+  if (T_out) *T_out = T;
+  // run length 16 never beat 8 on MI355X (T = 4, 8 slower; 16, 32 equal)
+  if (run_length_out) *run_length_out = 8;
+  if (x_staging_out) *x_staging_out = (op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX) ? 2 : 1;
+  return BGB_OK;
+}
+
 int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
                          double *cost4_us) {
   bgb::clear_error();

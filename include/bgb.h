@@ -236,6 +236,14 @@ int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colin
 int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int allowed_mask,
                          int *T_out, double *cost4_us);
 
+/* The whole per-graph kernel configuration for an op family: T (as
+ * bgb_autotune_tiledim), the run length per lane group (bgb_set_run_length;
+ * 8 for every measured case on MI355X) and the packed-x placement
+ * (bgb_set_x_staging: 1 = gathered from memory, 2 = staged in LDS; bit ops
+ * only, when x fits and T <= 16, else 1). Any output pointer may be NULL. */
+int bgb_autotune(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
+                 int *run_length_out, int *x_staging_out, double *cost4_us);
+
 /* ------------------------------------------------------------------------ *
  * Host graph generators and CSR normalisation (no reference counterpart    *
  * except the normalisation, readMtx.hpp:152-306: symmetrise, drop           *
@@ -248,6 +256,14 @@ typedef struct bgb_csr bgb_csr; /* opaque host CSR (int32 indices) */
 int bgb_csr_from_coo(int nrows, int ncols, long long nedges, const int *h_src,
                      const int *h_dst, int undirected, int drop_self_loops,
                      bgb_csr **out);
+/* MatrixMarket coordinate file (real / integer / pattern; values unused) ->
+ * normalised CSR, as the reference's readMtx + coo2csr (readMtx.hpp:353-422,
+ * test_bin_bin_full.cu:139-160): directed 0 = undirected iff the banner says
+ * symmetric (the drivers' setting), 1 = directed, 2 = undirected; self-loops
+ * and duplicates dropped. BGB_ERR_ARG if the file cannot be opened,
+ * BGB_ERR_FORMAT for a bad banner / size line, a short file or an entry out
+ * of range (where the reference exits or reads on). */
+int bgb_read_mtx(const char *path, int directed, bgb_csr **out);
 /* Erdos-Renyi G(n, m): exactly m distinct undirected edges, no self-loops. */
 int bgb_gen_er(int n, long long m, uint64_t seed, bgb_csr **out);
 /* R-MAT(scale, edge_factor*2^scale edges, a, b, c, d = 1-a-b-c), no noise or

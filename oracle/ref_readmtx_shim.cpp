@@ -1,7 +1,9 @@
 // Exposes the reference's own CSR construction (removeSelfloop + customSort +
 // coo2csr from bmv_eval/backend/readMtx.hpp:56-306, compiled where it lies in
-// the reference tree) through a C function, so the oracle's restatement can be
-// checked against it. Test infrastructure only; built into oracle/_ref/.
+// the reference tree) and its MatrixMarket reader (readMtx, :353-422, over
+// mmio.hpp) through C functions, so the oracle's restatement and the library's
+// reader can be checked against them. Test infrastructure only; built into
+// oracle/_ref/.
 #include <vector>
 
 #include "readMtx.hpp"
@@ -23,5 +25,26 @@ extern "C" int ref_normalize(int nrows, int ncols, long long nedges, const int *
   std::vector<float> vals(nvals > 0 ? nvals : 1);
   coo2csr<float>(rowptr, colind, vals.data(), r, c, v, nrows, ncols);
   *nnz = nvals;
+  return 0;
+}
+
+// The reference driver's path (test_bin_bin_full.cu:139-160): readMtx with
+// directed = 0 and no .dat name, then coo2csr. rowptr / colind must hold
+// nrows+1 / 2*entries ints; returns -1 where readMtx would not return.
+extern "C" int ref_read_mtx(const char *path, int directed, int *nrows, int *ncols, long long *nnz, int *rowptr,
+                            int *colind, long long cap) {
+  std::vector<int> r, c;
+  std::vector<float> v;
+  int M = 0, N = 0, nv = 0;
+  r.reserve(2 * cap + 1);
+  c.reserve(2 * cap + 1);
+  v.reserve(2 * cap + 1);
+  readMtx<float>(path, &r, &c, &v, &M, &N, &nv, directed, false, nullptr);
+  if ((long long)r.size() > cap) return -1;
+  std::vector<float> vals(r.size() > 0 ? r.size() : 1);
+  coo2csr<float>(rowptr, colind, vals.data(), r, c, v, M, N);
+  *nrows = M;
+  *ncols = N;
+  *nnz = (long long)r.size();
   return 0;
 }

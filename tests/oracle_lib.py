@@ -180,6 +180,20 @@ class RefReadMtx:
         f = self.L.ref_normalize
         f.restype, f.argtypes = ci, [ci, ci, ll, vp, vp, ci, vp, vp, vp]
 
+    def read_mtx(self, path, directed=0, cap=1 << 20):
+        """The reference driver's path: readMtx (directed) + coo2csr."""
+        f = self.L.ref_read_mtx
+        vp, ci, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+        f.restype = ci
+        f.argtypes = [ctypes.c_char_p, ci, vp, vp, vp, vp, vp, ll]
+        M, N, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        rp = np.zeros(cap + 1, np.int32)
+        col = np.zeros(2 * cap + 1, np.int32)
+        st = f(os.fsencode(path), int(directed), ctypes.byref(M), ctypes.byref(N), ctypes.byref(nnz), _p(rp),
+               _p(col), cap)
+        assert st == 0
+        return M.value, N.value, rp[: M.value + 1].copy(), col[: nnz.value].copy()
+
     def normalize(self, nrows, ncols, src, dst, undirected=True):
         src = np.ascontiguousarray(src, np.int32)
         dst = np.ascontiguousarray(dst, np.int32)

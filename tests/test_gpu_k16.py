@@ -82,3 +82,28 @@ def test_k16_persistent_batch(orc, dev):
     batch.run()
     for i, (csr, X64, T) in enumerate(refs):
         check(orc, csr, batch.result(i), X64, f"batch graph {i} n={csr.nrows} T={T}")
+
+
+@pytest.mark.timeout(600)
+def test_k16_batch_beyond_2e32_work_items(orc, dev):
+    """A batch whose carry fix-up covers more than 2^32 values (three R-MAT scale-22
+    graphs at T=16: ~8.6 M chunks x 512): the fix-up walks its range with a capped
+    grid, since a dispatch's work-item count is 32 bits."""
+    from bgb import b2sr as B
+    from bgb import graph
+    mats, xbs, refs = [], [], []
+    chunks = 0
+    for i in range(3):
+        csr = graph.rmat(22, 16, seed=40 + i)
+        A = B.B2SR.from_csr(csr, 16, dev)
+        X, X64 = rhs(csr.nrows, 200 + i, dev)
+        mats.append(A)
+        xbs.append(B.k16_pack_rhs(X, A.nblockrows * 16))
+        refs.append((csr, X64))
+        chunks += -(-A.nblocks // 32)
+    assert chunks * 512 > 2 ** 32
+    log(f"[case] k16 batch of 3 R-MAT 22 graphs: {chunks} chunks")
+    batch = B.K16Batch(mats, xbs)
+    batch.run()
+    for i, (csr, X64) in enumerate(refs):
+        check(orc, csr, batch.result(i), X64, f"rmat22 #{i}")

@@ -173,3 +173,15 @@ def test_tuning_knobs_validate_and_round_trip():
     finally:
         L.bgb_set_x_staging(16, 0)
         L.bgb_set_run_length(16, 8)
+
+
+def test_autotune_reports_the_whole_configuration():
+    # R-MAT 16: the packed x fits in LDS at every T <= 16, so the bit ops stage it
+    csr = graph.rmat(16, 8, seed=3)
+    c = graph.autotune(csr, "bin_bin")
+    assert c["T"] in (4, 8, 16, 32) and c["run_length"] == 8
+    assert c["x_staging"] == (2 if c["T"] <= 16 else 1)
+    assert c["T"] == graph.autotune_tiledim(csr, "bin_bin")[0]
+    # FP ops never stage a bit vector; K=16 picks from 16 / 32 only
+    assert graph.autotune(csr, "bin_full_full")["x_staging"] == 1
+    assert graph.autotune(csr, "k16")["T"] in (16, 32)
