@@ -33,9 +33,9 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kK = 16;        // right-hand sides
-constexpr int kChunk = 32;    // tiles per wave
+constexpr int kChunk = 256;   // tiles per wave
 constexpr int kWaves = 4;     // waves per workgroup
-constexpr int kUnroll = 4;    // tiles loaded ahead
+constexpr int kUnroll = 4;    // tiles per pipelined group
 
 // 4 bits (bit 3 = first column) -> four FP16 values 0.0 / 1.0.
 __device__ __forceinline__ half4_t nib_to_h4(uint32_t nib) {
@@ -83,21 +83,46 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
   if (k0 == 0)
     for (int r = 0; r < I; ++r) k16_flush<T>(G, r, lane, Z);
   int e = rowptr[I + 1];
-  for (long long p0 = k0; p0 < k1; p0 += kUnroll) {
-    uint32_t w[kUnroll][MH];
-    uint2 xv[kUnroll][MH];
+  // Groups of kUnroll tiles, software-pipelined: the block columns and row
+  // words of group g+1 are in flight while group g loads its X blocks and runs
+  // its MFMAs (one exposed memory latency per group instead of two).
+  uint32_t w[kUnroll][MH], wn[kUnroll][MH];
+  int J[kUnroll], Jn[kUnroll];
+  auto load_tiles = [&](long long p0, uint32_t (&ww)[kUnroll][MH], int (&jj)[kUnroll]) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const long long p = p0 + u;
-      if (p < k1) {
-        const int J = G.colind[p];
+      jj[u] = 0;
 #pragma unroll
-        for (int h = 0; h < MH; ++h) {
-          w[u][h] = A[p * T + h * 16 + m];
-          xv[u][h] = G.xb[((long long)J * MH + h) * 64 + lane];
-        }
+      for (int h = 0; h < MH; ++h) ww[u][h] = 0;
+      if (p < k1) {
+        jj[u] = G.colind[p];
+#pragma unroll
+        for (int h = 0; h < MH; ++h) ww[u][h] = A[p * T + h * 16 + m];
       }
     }
+  };
+  load_tiles(k0, w, J);
+  for (long long p0 = k0; p0 < k1; p0 += kUnroll) {
+    // X rows of a 4-column group only matter where the tile has a set bit in
+    // those columns (the A operand is 0 elsewhere): the tile's column mask is
+    // the OR of its row words, which sit in one 16-lane DPP row (lane m holds
+    // row m), so lanes of groups with no set bit load nothing.
+    uint2 xv[kUnroll][MH];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      uint32_t cols = 0;
+#pragma unroll
+      for (int h = 0; h < MH; ++h) cols |= w[u][h];
+      cols = row_or16(cols);
+#pragma unroll
+      for (int h = 0; h < MH; ++h) {
+        const uint32_t grp = (cols >> (T - 4 - (h * 16 + 4 * kq))) & 0xFu;
+        xv[u][h] = (p0 + u < k1 && grp) ? G.xb[((long long)J[u] * MH + h) * 64 + lane] : make_uint2(0u, 0u);
+      }
+    }
+    const bool more = p0 + kUnroll < k1;  // wave-uniform
+    if (more) load_tiles(p0 + kUnroll, wn, Jn);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const long long p = p0 + u;
@@ -124,6 +149,14 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
             e = I < G.nbr ? rowptr[I + 1] : INT_MAX;
           }
         }
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        J[u] = Jn[u];
+#pragma unroll
+        for (int h = 0; h < MH; ++h) w[u][h] = wn[u][h];
       }
     }
   }

@@ -374,6 +374,16 @@ __device__ __forceinline__ uint32_t dpp_shr(uint32_t v, uint32_t fill) {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x110 + D, 0xF, 0xF, false);
 }
 
+// OR of v over the lane's 16-lane row (DPP row_ror 1, 2, 4, 8: every lane of
+// the row ends with the whole row's OR).
+__device__ __forceinline__ uint32_t row_or16(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+  return v;
+}
+
 // Value of lane (lane - D) across the wave (lanes < D keep their own value),
 // DPP for in-row distances, ds_bpermute otherwise.
 template <int D>
