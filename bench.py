@@ -59,7 +59,7 @@ def measured_traffic(workload, T):
 
 
 def bmv_bytes(T, nbr, nblocks):
-    wb = {4: 1, 8: 1, 16: 2, 32: 4}[T]
+    wb = {4: 1, 8: 1, 16: 2, 32: 4, 64: 8}[T]
     x = -(-nbr // T) * T * wb
     y = 4 * nbr * T
     return 4 * (nbr + 1) + 4 * nblocks + nblocks * T * wb + x + y
@@ -447,13 +447,13 @@ def main():
     from bgb import graph
 
     if args.run_length:
-        for T in (4, 8, 16, 32):
+        for T in (4, 8, 16, 32, 64):
             bgb.call("bgb_set_run_length", T, args.run_length)
-    run_lengths = {T: bgb.lib().bgb_get_run_length(T) for T in (4, 8, 16, 32)}
+    run_lengths = {T: bgb.lib().bgb_get_run_length(T) for T in (4, 8, 16, 32, 64)}
     if args.x_staging >= 0:
-        for T in (4, 8, 16, 32):
+        for T in (4, 8, 16, 32, 64):
             bgb.call("bgb_set_x_staging", T, args.x_staging)
-    x_staging = {T: bgb.lib().bgb_get_x_staging(T) for T in (4, 8, 16, 32)}
+    x_staging = {T: bgb.lib().bgb_get_x_staging(T) for T in (4, 8, 16, 32, 64)}
     cfg = CONFIGS[args.config]
     log(f"[bench] generating R-MAT scale {cfg['scale']} ef {cfg['edge_factor']}")
     t0 = time.perf_counter()

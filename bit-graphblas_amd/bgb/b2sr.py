@@ -11,9 +11,9 @@ import torch
 from ._lib import call, lib
 
 # Raw storage dtypes for the B2SR words (bit patterns; view as unsigned on host).
-WORD_TORCH = {4: torch.uint8, 8: torch.uint8, 16: torch.int16, 32: torch.int32}
-WORD_NUMPY = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32}
-TILE_DIMS = (4, 8, 16, 32)
+WORD_TORCH = {4: torch.uint8, 8: torch.uint8, 16: torch.int16, 32: torch.int32, 64: torch.int64}
+WORD_NUMPY = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+TILE_DIMS = (4, 8, 16, 32, 64)
 
 
 def _stream():

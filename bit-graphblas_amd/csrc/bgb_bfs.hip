@@ -494,7 +494,8 @@ int bfs_plain_impl(const void *A, const int *rowptr, const int *colind, int nbr,
 
 int bfs_check(int T, const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
               int *level, int *h_nlevels) {
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T) && T <= 32, BGB_ERR_TILEDIM, "the BFS drivers take tile dims 4, 8, 16, 32 (got %d)",
+              T);
   BGB_REQUIRE(rowptr && level && h_nlevels, BGB_ERR_ARG, "null argument");
   BGB_REQUIRE(nblocks == 0 || (A && colind), BGB_ERR_ARG, "null matrix");
   BGB_REQUIRE(n >= 1 && nbr == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");

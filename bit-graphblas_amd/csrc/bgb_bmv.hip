@@ -68,9 +68,9 @@ constexpr int kWavesPerBlock = 4;
 // Tiles per lane-group run (S), per tile dim (index log2(T)-2): 8 or 16.
 // Longer runs put more bytes in flight per wave, which matters for the small
 // tiles; set with bgb_set_run_length.
-int g_run_length[4] = {8, 8, 8, 8};
+int g_run_length[5] = {8, 8, 8, 8, 8};
 // Packed-x staging per tile dim: 0 automatic, 1 never, 2 whenever it fits.
-int g_x_staging[4] = {0, 0, 0, 0};
+int g_x_staging[5] = {0, 0, 0, 0, 0};
 
 template <int OP>
 struct AccOf {
@@ -92,15 +92,16 @@ __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
   constexpr int R = Tile<T>::R, L = Tile<T>::L;
   using W = typename Tile<T>::W;
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
-    uint32_t bits = 0;
+    using Wd = Word<T>;
+    Wd bits = 0;
 #pragma unroll
-    for (int i = 0; i < R; ++i) bits |= (v[i] != 0u ? 1u : 0u) << (T - 1 - (q * R + i));
+    for (int i = 0; i < R; ++i) bits |= (Wd)(v[i] != 0u ? 1u : 0u) << (T - 1 - (q * R + i));
 #pragma unroll
-    for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
+    for (int m = 1; m < L; m <<= 1) bits |= __shfl_xor(bits, m);
     if (q == 0) {
       W *C = static_cast<W *>(a.C);
-      if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[I];
-      else if (a.accumulate) bits |= (uint32_t)C[I];
+      if constexpr (OP == OP_MASKED) bits &= ~(Wd) static_cast<const W *>(a.mask)[I];
+      else if (a.accumulate) bits |= (Wd)C[I];
       C[I] = (W)bits;
     }
   } else {
@@ -125,9 +126,9 @@ __device__ __forceinline__ void flush_row(const BmvArgs &a, int I, int q,
 // there (word J in byte J/2, high nibble for odd J), T >= 8 the words as
 // they are.
 template <int T>
-__device__ __forceinline__ uint32_t lds_xword(const uint8_t *xl, int J) {
+__device__ __forceinline__ Word<T> lds_xword(const uint8_t *xl, int J) {
   if constexpr (T == 4) return (uint32_t)(xl[J >> 1] >> ((J & 1) * 4)) & 0xFu;
-  else return (uint32_t) reinterpret_cast<const typename Tile<T>::W *>(xl)[J];
+  else return (Word<T>) reinterpret_cast<const typename Tile<T>::W *>(xl)[J];
 }
 
 // What one chunk needs before its walk: each lane group's tile run, the
@@ -175,12 +176,12 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   const uint32_t(&F)[S][NW] = in.F;
   const int(&J)[S] = in.J;
 
-  uint32_t X[S];
+  Word<T> X[S];
   if constexpr (!is_full(OP)) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if constexpr (XL) X[s] = (active && s < n) ? lds_xword<T>(xl, J[s]) : 0u;
-      else X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(a.B)[J[s]] : 0u;
+      if constexpr (XL) X[s] = (active && s < n) ? lds_xword<T>(xl, J[s]) : (Word<T>)0;
+      else X[s] = (active && s < n) ? (Word<T>) static_cast<const W *>(a.B)[J[s]] : (Word<T>)0;
     }
   }
 
@@ -248,7 +249,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         // ascending columns, adding 0 for clear bits: the CSR summation order
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          const uint32_t w = row_word<T>(F[s], i);
+          const Word<T> w = row_word<T>(F[s], i);
 #pragma unroll
           for (int c = 0; c < T; ++c) acc[i] += ((w >> (T - 1 - c)) & 1u) ? XS[s][c] : 0.0f;
         }
@@ -256,27 +257,27 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         const float *__restrict__ x = static_cast<const float *>(a.B) + (long long)J[s] * T;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          uint32_t w = row_word<T>(F[s], i);
+          Word<T> w = row_word<T>(F[s], i);
           while (w) {
-            const int h = 31 - __clz(w);
+            const int h = top_bit(w);
             acc[i] += x[T - 1 - h];
-            w &= ~(1u << h);
+            w &= ~((Word<T>)1 << h);
           }
         }
       } else if constexpr (OP == OP_FULL_BF16) {
         const uint16_t *__restrict__ x = static_cast<const uint16_t *>(a.B) + (long long)J[s] * T;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          uint32_t w = row_word<T>(F[s], i);
+          Word<T> w = row_word<T>(F[s], i);
           while (w) {
-            const int h = 31 - __clz(w);
+            const int h = top_bit(w);
             acc[i] += bf16_to_f32(x[T - 1 - h]);
-            w &= ~(1u << h);
+            w &= ~((Word<T>)1 << h);
           }
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < R; ++i) acc[i] += __popc(row_word<T>(F[s], i) & X[s]);
+        for (int i = 0; i < R; ++i) acc[i] += bit_count(row_word<T>(F[s], i) & X[s]);
       }
       const int pos = (int)(pa + s) + 1;
       if (pos == e) {
@@ -479,15 +480,19 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
     const int key = a.carry_key[tid];
     if (key < 0) return;
     const Acc *cv = static_cast<const Acc *>(a.carry_val) + tid * T;
-    uint32_t bits = 0;
+    Word<T> bits = 0;
 #pragma unroll
-    for (int r = 0; r < T; ++r) bits |= (cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
-    if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[key];
+    for (int r = 0; r < T; ++r) bits |= (Word<T>)(cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
+    if constexpr (OP == OP_MASKED) bits &= ~(Word<T>) static_cast<const W *>(a.mask)[key];
     if (bits == 0) return;
-    // OR into the aligned dword holding word `key` (W is 1, 2 or 4 bytes).
-    const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(a.C) + key);
-    const int shift = (int)(addr & 3) * 8;
-    atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), bits << shift);
+    if constexpr (T == 64) {
+      atomicOr(reinterpret_cast<unsigned long long *>(static_cast<W *>(a.C) + key), (unsigned long long)bits);
+    } else {
+      // OR into the aligned dword holding word `key` (W is 1, 2 or 4 bytes).
+      const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(a.C) + key);
+      const int shift = (int)(addr & 3) * 8;
+      atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), (uint32_t)bits << shift);
+    }
   } else {
     typedef Acc Acc4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
@@ -619,7 +624,7 @@ template <int OP>
 int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, const int *colind,
               int nbr, int nblocks, int accumulate, const void *mask, bgb_stream_t stream) {
   clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(nbr >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
   BGB_REQUIRE(C && rowptr, BGB_ERR_ARG, "null output or rowptr");
   BGB_REQUIRE(nblocks == 0 || (A && B && colind), BGB_ERR_ARG, "null matrix / vector");
@@ -647,7 +652,8 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
     case 4: return launch_bmv<4, OP>(a, s);
     case 8: return launch_bmv<8, OP>(a, s);
     case 16: return launch_bmv<16, OP>(a, s);
-    default: return launch_bmv<32, OP>(a, s);
+    case 32: return launch_bmv<32, OP>(a, s);
+    default: return launch_bmv<64, OP>(a, s);
   }
 }
 
@@ -781,7 +787,7 @@ extern "C" {
 
 int bgb_set_run_length(int T, int S) {
   clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(S == 8 || S == 16, BGB_ERR_ARG, "run length must be 8 or 16 (got %d)", S);
   g_run_length[ilog2_tiledim(T) - 2] = S;
   return BGB_OK;
@@ -791,7 +797,7 @@ int bgb_get_run_length(int T) { return valid_tiledim(T) ? g_run_length[ilog2_til
 
 int bgb_set_x_staging(int T, int mode) {
   clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "x staging mode must be 0, 1 or 2 (got %d)", mode);
   g_x_staging[ilog2_tiledim(T) - 2] = mode;
   return BGB_OK;

@@ -62,6 +62,6 @@ inline unsigned capped_blocks(long long n, int block) {
   return (unsigned)(b < 1 ? 1 : (b > (1LL << 20) ? (1LL << 20) : b));
 }
 
-inline int ilog2_tiledim(int T) { return T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : 5; }
+inline int ilog2_tiledim(int T) { return T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6; }
 
 }  // namespace bgb

@@ -15,8 +15,10 @@ namespace bgb {
 int set_error(int status, const char *fmt, ...);
 void clear_error();
 
-inline bool valid_tiledim(int T) { return T == 4 || T == 8 || T == 16 || T == 32; }
-inline int word_bytes(int T) { return T == 32 ? 4 : (T == 16 ? 2 : (T == 4 || T == 8 ? 1 : 0)); }
+inline bool valid_tiledim(int T) { return T == 4 || T == 8 || T == 16 || T == 32 || T == 64; }
+inline int word_bytes(int T) {
+  return T == 64 ? 8 : T == 32 ? 4 : (T == 16 ? 2 : (T == 4 || T == 8 ? 1 : 0));
+}
 inline long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
 
 // splitmix64 finaliser (Steele, Lea, Flood 2014). The random value for index i

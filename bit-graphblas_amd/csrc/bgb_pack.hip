@@ -155,6 +155,24 @@ __global__ void tile_fill_kernel(const uint64_t *__restrict__ keys, const int *_
 }
 
 // One wave packs 64 consecutive entries of x with a 64-bit ballot.
+// T = 64: one thread per (tile, row) word. The tile's nonzeros are sorted by
+// (row, column), so the thread ORs the bits of its row's run; 64 words of 8 B
+// per tile would not fit one thread's registers.
+__global__ void tile_fill64_kernel(const uint64_t *__restrict__ keys, const int *__restrict__ head_pos,
+                                   long long nwords, uint64_t *__restrict__ val) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nwords) return;
+  const long long tile = t >> 6;
+  const int r = (int)(t & 63);
+  const int s = head_pos[tile], e = head_pos[tile + 1];
+  uint64_t w = 0;
+  for (int k = s; k < e; ++k) {
+    const uint32_t rc = (uint32_t)(keys[k] & 0xFFFu);  // 6 row bits, 6 column bits
+    if ((int)(rc >> 6) == r) w |= 1ull << (63 - (rc & 63));
+  }
+  val[t] = w;
+}
+
 template <int T>
 __global__ void pack_vec_kernel(const float *__restrict__ x, long long n, long long nwords,
                                 void *__restrict__ xb) {
@@ -167,6 +185,10 @@ __global__ void pack_vec_kernel(const float *__restrict__ x, long long n, long l
   const uint64_t rb = __builtin_bitreverse64(m);  // x[base + l] now at bit 63 - l
   const long long word = base / T + lane;
   if (lane < PER && word < nwords) {
+    if constexpr (T == 64) {
+      static_cast<uint64_t *>(xb)[word] = rb;
+      return;
+    }
     const uint64_t v = (rb >> (64 - T * (lane + 1))) & ((T == 32) ? 0xffffffffull : ((1ull << T) - 1));
     if constexpr (T == 32) static_cast<uint32_t *>(xb)[word] = (uint32_t)v;
     else if constexpr (T == 16) static_cast<uint16_t *>(xb)[word] = (uint16_t)v;
@@ -177,7 +199,7 @@ __global__ void pack_vec_kernel(const float *__restrict__ x, long long n, long l
 unsigned grid_for(long long n, int block = kBlock) { return (unsigned)ceil_div(std::max(n, 1LL), block); }
 
 int check_common(int T, int nrows, int ncols, int nnz, const int *rowptr, const int *colind) {
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, BGB_ERR_ARG, "negative size");
   BGB_REQUIRE(rowptr && (nnz == 0 || colind), BGB_ERR_ARG, "null CSR array");
   return BGB_OK;
@@ -295,7 +317,11 @@ int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
     case 4: tile_fill_kernel<4><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
     case 8: tile_fill_kernel<8><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
     case 16: tile_fill_kernel<16><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
-    default: tile_fill_kernel<32><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+    case 32: tile_fill_kernel<32><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+    default:
+      tile_fill64_kernel<<<grid_for(64LL * nblocks), kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(),
+                                                                     64LL * nblocks, static_cast<uint64_t *>(d_bsrVal));
+      break;
   }
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
@@ -303,7 +329,7 @@ int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
 
 int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream) {
   clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(n >= 0 && d_xb && (n == 0 || d_x), BGB_ERR_ARG, "bad vector arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long nbr = ceil_div(n, T);
@@ -315,7 +341,8 @@ int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream
     case 4: pack_vec_kernel<4><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
     case 8: pack_vec_kernel<8><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
     case 16: pack_vec_kernel<16><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-    default: pack_vec_kernel<32><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+    case 32: pack_vec_kernel<32><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+    default: pack_vec_kernel<64><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
   }
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;

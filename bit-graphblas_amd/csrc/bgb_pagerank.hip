@@ -124,11 +124,15 @@ __global__ void row_degree_kernel(const void *A, const int *rowptr, int nbr, int
   const int I = (int)(v / T), r = (int)(v % T);
   int d = 0;
   for (int k = rowptr[I]; k < rowptr[I + 1]; ++k) {
-    uint32_t w;
-    if (T == 32) w = static_cast<const uint32_t *>(A)[(long long)k * T + r];
-    else if (T == 16) w = static_cast<const uint16_t *>(A)[(long long)k * T + r];
-    else w = static_cast<const uint8_t *>(A)[(long long)k * T + r];
-    d += __popc(w);
+    if constexpr (T == 64) {
+      d += __popcll(static_cast<const uint64_t *>(A)[(long long)k * T + r]);
+    } else {
+      uint32_t w;
+      if (T == 32) w = static_cast<const uint32_t *>(A)[(long long)k * T + r];
+      else if (T == 16) w = static_cast<const uint16_t *>(A)[(long long)k * T + r];
+      else w = static_cast<const uint8_t *>(A)[(long long)k * T + r];
+      d += __popc(w);
+    }
   }
   deg[v] = d;
 }
@@ -151,7 +155,8 @@ int pagerank_impl(int T, const void *A, const int *rowptr, const int *colind, in
     case 4: row_degree_kernel<4><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
     case 8: row_degree_kernel<8><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
     case 16: row_degree_kernel<16><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
-    default: row_degree_kernel<32><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 32: row_degree_kernel<32><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    default: row_degree_kernel<64><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
   }
   pr_init_kernel<Acc, BF><<<blocks, kPrBlock, 0, s>>>(pr_out, deg.as<int>(), n, npad, x.ptr, pdang.as<Acc>());
   pr_finish_kernel<Acc><<<1, kPrBlock, 0, s>>>(nullptr, pdang.as<Acc>(), (int)blocks, nullptr, dtot.as<Acc>());
@@ -188,7 +193,7 @@ extern "C" int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const i
                             int nblocks, int n, float damping, int iters, int mode, float *d_pr,
                             double *h_residuals, bgb_stream_t stream) {
   clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32}", T);
+  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
   BGB_REQUIRE(d_rowptr && d_pr && (nblocks == 0 || (d_A && d_colind)), BGB_ERR_ARG, "null argument");
   BGB_REQUIRE(n >= 1 && nblockrows == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");
   BGB_REQUIRE(iters >= 0 && damping >= 0.0f && damping <= 1.0f, BGB_ERR_ARG, "bad iters / damping");

@@ -91,10 +91,20 @@ void csr2bsr_batch_32(const int *h_csrRowPtr, const int *h_csrColInd, int nrows,
   batch(32, h_csrRowPtr, h_csrColInd, nrows, ncols, nnz, bsrRowPtr, bsrColInd, bsrVal, blocksize, nblocks);
 }
 
+void csr2bsr_batch_64(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz, int *bsrRowPtr,
+                      int *bsrColInd, ullong *bsrVal, int blocksize, int, int nblocks) {
+  batch(64, h_csrRowPtr, h_csrColInd, nrows, ncols, nnz, bsrRowPtr, bsrColInd, bsrVal, blocksize, nblocks);
+}
+
 void ToBit4Row(const float *A, uchar *B, int nblockrows) { tobit(4, A, B, nblockrows); }
 void ToBit8Row(const float *A, uchar *B, int nblockrows) { tobit(8, A, B, nblockrows); }
 void ToBit16Row(const float *A, ushort *B, int nblockrows) { tobit(16, A, B, nblockrows); }
 void ToBit32Row(const float *A, unsigned *B, int nblockrows) { tobit(32, A, B, nblockrows); }
+void ToBit64Row(const float *A, ullong *B, int nblockrows) { tobit(64, A, B, nblockrows); }
+void bmv64_bin_bin_bin(const ullong *A, const ullong *B, ullong *C, const int *rp, const int *ci, int nbr) { bbb(64, A, B, C, rp, ci, nbr); }
+void bmv64_bin_bin_full(const ullong *A, const ullong *B, float *C, const int *rp, const int *ci, int nbr) { bbf(64, A, B, C, rp, ci, nbr); }
+void bmv64_bin_full_full(const ullong *A, const float *B, float *C, const int *rp, const int *ci, int nbr) { bff(64, A, B, C, rp, ci, nbr); }
+void bmv64_bin_bin_bin_masked(const ullong *A, const ullong *B, ullong *C, const int *rp, const int *ci, int nbr, const ullong *m) { bbm(64, A, B, C, rp, ci, nbr, m); }
 
 void bmv4_bin_bin_bin(const uchar *A, const uchar *B, uchar *C, const int *rp, const int *ci, int nbr) { bbb(4, A, B, C, rp, ci, nbr); }
 void bmv8_bin_bin_bin(const uchar *A, const uchar *B, uchar *C, const int *rp, const int *ci, int nbr) { bbb(8, A, B, C, rp, ci, nbr); }

@@ -6,6 +6,8 @@
 //   T=8 : 1 lane,  8 rows, 2 dwords -- byte r = row r, col c at bit 7-c
 //   T=16: 2 lanes, 8 rows, 4 dwords -- half-word i = row 8q+i, col c at bit 15-c
 //   T=32: 8 lanes, 4 rows, 4 dwords -- dword i = row 4q+i, col c at bit 31-c
+//   T=64: 16 lanes, 4 rows, 8 dwords -- qword i = row 4q+i, col c at bit 63-c
+// Row words are 32-bit (Word) up to T = 32 and 64-bit at T = 64.
 // Every lane therefore loads 4, 8 or 16 contiguous bytes per tile and a wave
 // streams 64/L tiles per instruction.
 #pragma once
@@ -13,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace bgb {
 
@@ -38,6 +41,21 @@ struct Tile<32> {
   using W = uint32_t;
   static constexpr int L = 8, R = 4, NW = 4, LG = 5;
 };
+template <>
+struct Tile<64> {
+  using W = uint64_t;
+  static constexpr int L = 16, R = 4, NW = 8, LG = 6;
+};
+
+// Register type of one row word (and of a packed-x word).
+template <int T>
+using Word = typename std::conditional<T == 64, uint64_t, uint32_t>::type;
+
+__device__ __forceinline__ int bit_count(uint32_t w) { return __popc(w); }
+__device__ __forceinline__ int bit_count(uint64_t w) { return __popcll(w); }
+// Index of the highest set bit of a non-zero word.
+__device__ __forceinline__ int top_bit(uint32_t w) { return 31 - __clz((int)w); }
+__device__ __forceinline__ int top_bit(uint64_t w) { return 63 - __clzll((long long)w); }
 
 // 16-byte non-temporal load (streamed data that is read once per pass).
 __device__ __forceinline__ uint4 load_nt16(const void *p) {
@@ -48,11 +66,12 @@ __device__ __forceinline__ uint4 load_nt16(const void *p) {
 
 // Row i (0 <= i < R) of a lane fragment, as a T-bit word.
 template <int T>
-__device__ __forceinline__ uint32_t row_word(const uint32_t (&F)[Tile<T>::NW], int i) {
+__device__ __forceinline__ Word<T> row_word(const uint32_t (&F)[Tile<T>::NW], int i) {
   if constexpr (T == 4) return (F[0] >> (8 * i)) & 0xFu;
   else if constexpr (T == 8) return (F[i >> 2] >> (8 * (i & 3))) & 0xFFu;
   else if constexpr (T == 16) return (F[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
-  else return F[i];
+  else if constexpr (T == 32) return F[i];
+  else return (uint64_t)F[2 * i] | ((uint64_t)F[2 * i + 1] << 32);
 }
 
 // Fragment of tile p for lane q of its group.
@@ -65,7 +84,7 @@ __device__ __forceinline__ void load_frag(const void *__restrict__ A, long long 
     const uint2 v = static_cast<const uint2 *>(A)[p];
     F[0] = v.x;
     F[1] = v.y;
-  } else {
+  } else if constexpr (T <= 32) {
     constexpr int L = Tile<T>::L;
     // T = 32 streams 128 B per tile: non-temporal loads keep the tile stream
     // from displacing x, rowptr and the outputs in L2 (measured faster at
@@ -76,6 +95,18 @@ __device__ __forceinline__ void load_frag(const void *__restrict__ A, long long 
     F[1] = v.y;
     F[2] = v.z;
     F[3] = v.w;
+  } else {
+    // T = 64: 512 B per tile, 32 B (rows 4q..4q+3) per lane, non-temporal
+    const uint4 *src = static_cast<const uint4 *>(A) + (p * 16 + q) * 2;
+    const uint4 v0 = load_nt16(src), v1 = load_nt16(src + 1);
+    F[0] = v0.x;
+    F[1] = v0.y;
+    F[2] = v0.z;
+    F[3] = v0.w;
+    F[4] = v1.x;
+    F[5] = v1.y;
+    F[6] = v1.z;
+    F[7] = v1.w;
   }
 }
 
@@ -94,7 +125,7 @@ __device__ __forceinline__ void load_run(const void *__restrict__ A, const int *
 #pragma unroll
     for (int s = 0; s < S / 4; ++s) {
       int4 c;
-      if constexpr (T == 32) {
+      if constexpr (T >= 32) {
         const uint4 u = load_nt16(ci + s);
         c = make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
       } else {

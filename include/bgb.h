@@ -7,13 +7,17 @@
  * Device pointers are prefixed `d_`, host pointers `h_`.
  *
  * B2SR layout (identical to the reference's, bsrbmv.cu:28-272 and
- * test_bin_bin_full.cu:49-59): tile dim T in {4, 8, 16, 32},
+ * test_bin_bin_full.cu:49-59): tile dim T in {4, 8, 16, 32, 64},
  *   NBR          = ceil(nrows / T) block rows
  *   bsrRowPtr    int32[NBR + 1]
  *   bsrColInd    int32[nblocks]            ascending block column J per block row
  *   bsrVal       W_T[nblocks * T]          word k*T + r holds row r of tile k,
  *                                          column c at bit (T-1-c)
- *   W_T = uint8 (T=4, 8), uint16 (T=16), uint32 (T=32); for T=4 the high nibble is 0.
+ *   W_T = uint8 (T=4, 8), uint16 (T=16), uint32 (T=32), uint64 (T=64); for T=4 the
+ *   high nibble is 0. The reference packs T=64 (csr2bsr_batch_64, ToBit64Row/Col,
+ *   csr2bsr_batch_bsrbmv.cu:510-628, bsrbmv.cu:224-272) but has no T=64 BMV; here
+ *   every BMV operator, the vector packer and PageRank take T=64 too (the BFS
+ *   drivers take T <= 32, the K=16 MFMA path T = 16, 32).
  * Packed bit vector (ToBit{T}Row, bsrbmv.cu:65-222): W_T[ceil(NBR/T)*T] words,
  *   word J holds x[J*T + c] > 0 at bit (T-1-c); for T=4 one nibble per byte.
  * Bit output of bin_bin_bin: W_T[NBR], word I holds row I*T + r at bit (T-1-r).
@@ -34,7 +38,7 @@ typedef void *bgb_stream_t; /* hipStream_t */
 enum {
   BGB_OK = 0,
   BGB_ERR_ARG = 1,      /* null pointer, negative size, bad flag */
-  BGB_ERR_TILEDIM = 2,  /* T not in {4, 8, 16, 32} */
+  BGB_ERR_TILEDIM = 2,  /* T not in {4, 8, 16, 32, 64}, or not one the entry point takes */
   BGB_ERR_HIP = 3,      /* a HIP runtime call failed */
   BGB_ERR_ALLOC = 4,    /* device / host allocation failed */
   BGB_ERR_SIZE = 5,     /* a size exceeds the int32 index range or a stated limit */

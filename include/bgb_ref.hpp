@@ -22,6 +22,7 @@ namespace bgb_ref {
 
 typedef unsigned char uchar;
 typedef unsigned short ushort;
+typedef unsigned long long ullong;
 
 // Status of the last call on this thread (BGB_OK on success).
 int last_status();
@@ -30,8 +31,8 @@ int last_status();
 void csr2bsr_nnz(const int *csrRowPtr, const int *csrColInd, int nrows, int ncols, int nnz, int blocksize,
                  int *bsrRowPtr, int *nblocks);
 
-// csr2bsr_batch_{4,8,16,32} (csr2bsr_batch_bsrbmv.cu:16,140,263,386): host CSR
-// in, device B2SR out; bsrRowPtr already filled by the nnz pass.
+// csr2bsr_batch_{4,8,16,32,64} (csr2bsr_batch_bsrbmv.cu:16,140,263,386,510): host
+// CSR in, device B2SR out; bsrRowPtr already filled by the nnz pass.
 void csr2bsr_batch_4(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz,
                      int *bsrRowPtr, int *bsrColInd, uchar *bsrVal, int blocksize, int nblockrows, int nblocks);
 void csr2bsr_batch_8(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz,
@@ -41,13 +42,28 @@ void csr2bsr_batch_16(const int *h_csrRowPtr, const int *h_csrColInd, int nrows,
 void csr2bsr_batch_32(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz,
                       int *bsrRowPtr, int *bsrColInd, unsigned *bsrVal, int blocksize, int nblockrows,
                       int nblocks);
+void csr2bsr_batch_64(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz,
+                      int *bsrRowPtr, int *bsrColInd, ullong *bsrVal, int blocksize, int nblockrows,
+                      int nblocks);
 
-// ToBit{T}Row (bsrbmv.cu:65,117,163,206): x padded to nblockrows*T floats ->
+// ToBit{T}Row (bsrbmv.cu:65,117,163,206,253): x padded to nblockrows*T floats ->
 // ceil(nblockrows/T)*T packed words.
 void ToBit4Row(const float *A, uchar *B, int nblockrows);
 void ToBit8Row(const float *A, uchar *B, int nblockrows);
 void ToBit16Row(const float *A, ushort *B, int nblockrows);
 void ToBit32Row(const float *A, unsigned *B, int nblockrows);
+void ToBit64Row(const float *A, ullong *B, int nblockrows);
+
+// T = 64: the reference packs 64-bit tiles (ToBit64Col/Row, csr2bsr_batch_64)
+// but has no BMV for them; these follow the same contract as the T <= 32 ones.
+void bmv64_bin_bin_bin(const ullong *A, const ullong *B, ullong *C, const int *rowptr, const int *colind,
+                       int nblockrows);
+void bmv64_bin_bin_full(const ullong *A, const ullong *B, float *C, const int *rowptr, const int *colind,
+                        int nblockrows);
+void bmv64_bin_full_full(const ullong *A, const float *B, float *C, const int *rowptr, const int *colind,
+                         int nblockrows);
+void bmv64_bin_bin_bin_masked(const ullong *A, const ullong *B, ullong *C, const int *rowptr, const int *colind,
+                              int nblockrows, const ullong *mask);
 
 // bmv{T}_bin_bin_bin (bsrbmv.cu:277,437,508,579): C[I] |= bits.
 void bmv4_bin_bin_bin(const uchar *A, const uchar *B, uchar *C, const int *rowptr, const int *colind,

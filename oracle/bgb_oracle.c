@@ -10,12 +10,15 @@
  *                          (customSort), :248-306 (coo2csr)
  *   orc_b2sr_nnzb/_fill    cusparseXcsr2bsrNnz + cusparseScsr2bsr (textbook BSR,
  *                          row-major dense tiles) + ToBit{T}Col bit order
- *                          (bsrbmv.cu:33-203; column c at bit T-1-c), as driven
- *                          by csr2bsr_batch_{4,8,16,32} (csr2bsr_batch_bsrbmv.cu:34-133)
- *   orc_pack_vec           ToBit{T}Row (bsrbmv.cu:65-222) on x padded to NBR*T
+ *                          (bsrbmv.cu:33-272; column c at bit T-1-c), as driven
+ *                          by csr2bsr_batch_{4,8,16,32,64} (csr2bsr_batch_bsrbmv.cu:
+ *                          34-133, 510-628; T = 64 words are ullong)
+ *   orc_pack_vec           ToBit{T}Row (bsrbmv.cu:65-272) on x padded to NBR*T
  *   orc_b2sr_bmv_*         bmv{T}_bin_bin_bin / _bin_bin_full / _bin_full_full /
  *                          _bin_bin_bin_masked (bsrbmv.cu:277-1229), accumulate
- *                          semantics of the drivers (test_bin_bin_full.cu:311-353)
+ *                          semantics of the drivers (test_bin_bin_full.cu:311-353);
+ *                          at T = 64 (the reference packs 64-bit tiles but has no
+ *                          BMV for them) the same loops over 64-bit words
  *   orc_csr_*              textbook CSR SpMV over the (OR, AND), (+, x) and FP
  *                          semirings -- the "CPU CSR computation" the B2SR path
  *                          must match (the reference itself only compares with
@@ -32,19 +35,22 @@
 #include <stdlib.h>
 #include <string.h>
 
-static int wbytes(int T) { return T == 32 ? 4 : T == 16 ? 2 : 1; }
+static int wbytes(int T) { return T == 64 ? 8 : T == 32 ? 4 : T == 16 ? 2 : 1; }
 
-static uint32_t get_word(const void *p, int T, long long i) {
+/* W_T words: u8 (T = 4, 8), u16 (16), u32 (32), u64 (64) */
+static uint64_t get_word(const void *p, int T, long long i) {
+  if (T == 64) return ((const uint64_t *)p)[i];
   if (T == 32) return ((const uint32_t *)p)[i];
   if (T == 16) return ((const uint16_t *)p)[i];
   return ((const uint8_t *)p)[i];
 }
-static void set_word(void *p, int T, long long i, uint32_t v) {
-  if (T == 32) ((uint32_t *)p)[i] = v;
+static void set_word(void *p, int T, long long i, uint64_t v) {
+  if (T == 64) ((uint64_t *)p)[i] = v;
+  else if (T == 32) ((uint32_t *)p)[i] = (uint32_t)v;
   else if (T == 16) ((uint16_t *)p)[i] = (uint16_t)v;
   else ((uint8_t *)p)[i] = (uint8_t)v;
 }
-static int popc(uint32_t v) { return __builtin_popcount(v); }
+static int popc(uint64_t v) { return __builtin_popcountll(v); }
 
 /* ---------------- CSR normalisation (readMtx.hpp:152-306) ---------------- */
 typedef struct { int r, c; } rc_t;
@@ -147,9 +153,9 @@ void orc_b2sr_fill(int T, int nrows, int ncols, const int *rowptr, const int *co
           if (colind[e] / T == J) dense[r * T + colind[e] % T] = 1.0f;
       }
       for (int r = 0; r < T; ++r) {
-        uint32_t w = 0;
+        uint64_t w = 0;
         for (int c = 0; c < T; ++c)
-          if (dense[r * T + c] > 0) w |= 1u << (T - 1 - c);
+          if (dense[r * T + c] > 0) w |= 1ull << (T - 1 - c);
         set_word(bsrVal, T, (long long)k * T + r, w);
       }
     }
@@ -161,10 +167,10 @@ void orc_b2sr_fill(int T, int nrows, int ncols, const int *rowptr, const int *co
 /* ToBit{T}Row on x[0..n) zero-padded: nwords = ceil(NBR/T)*T words */
 void orc_pack_vec(int T, const float *x, int n, long long nwords, void *xb) {
   for (long long J = 0; J < nwords; ++J) {
-    uint32_t w = 0;
+    uint64_t w = 0;
     for (int c = 0; c < T; ++c) {
       long long i = J * T + c;
-      if (i < n && x[i] > 0) w |= 1u << (T - 1 - c);
+      if (i < n && x[i] > 0) w |= 1ull << (T - 1 - c);
     }
     set_word(xb, T, J, w);
   }
@@ -173,10 +179,10 @@ void orc_pack_vec(int T, const float *x, int n, long long nwords, void *xb) {
 /* packs a 0/1 byte per vertex the same way (for bit outputs) */
 void orc_pack_u8(int T, const uint8_t *bits, int n, long long nwords, void *out) {
   for (long long J = 0; J < nwords; ++J) {
-    uint32_t w = 0;
+    uint64_t w = 0;
     for (int c = 0; c < T; ++c) {
       long long i = J * T + c;
-      if (i < n && bits[i]) w |= 1u << (T - 1 - c);
+      if (i < n && bits[i]) w |= 1ull << (T - 1 - c);
     }
     set_word(out, T, J, w);
   }
@@ -186,12 +192,12 @@ void orc_pack_u8(int T, const uint8_t *bits, int n, long long nwords, void *out)
 void orc_b2sr_bmv_bin_bin_bin(int T, const void *A, const void *B, void *C, const int *rowptr,
                               const int *colind, int nbr) {
   for (int I = 0; I < nbr; ++I) {
-    uint32_t out = 0;
+    uint64_t out = 0;
     for (int r = 0; r < T; ++r) {
       int cnt = 0;
       for (int k = rowptr[I]; k < rowptr[I + 1]; ++k)
         cnt += popc(get_word(A, T, (long long)k * T + r) & get_word(B, T, colind[k]));
-      if (cnt > 0) out |= 1u << (T - 1 - r);
+      if (cnt > 0) out |= 1ull << (T - 1 - r);
     }
     set_word(C, T, I, get_word(C, T, I) | out); /* C[I] |= (bsrbmv.cu:308,612) */
   }
@@ -215,9 +221,9 @@ void orc_b2sr_bmv_bin_full_full(int T, const void *A, const float *B, float *C, 
     for (int r = 0; r < T; ++r) {
       float sum = 0.0f;
       for (int k = rowptr[I]; k < rowptr[I + 1]; ++k) {
-        uint32_t w = get_word(A, T, (long long)k * T + r);
+        uint64_t w = get_word(A, T, (long long)k * T + r);
         for (int c = 0; c < T; ++c)
-          if ((w >> (T - 1 - c)) & 1u) sum += B[(long long)colind[k] * T + c];
+          if ((w >> (T - 1 - c)) & 1ull) sum += B[(long long)colind[k] * T + c];
       }
       C[(long long)I * T + r] += sum;
     }
@@ -228,12 +234,12 @@ void orc_b2sr_bmv_bin_full_full(int T, const void *A, const float *B, float *C, 
 void orc_b2sr_bmv_masked(int T, const void *A, const void *B, void *C, const int *rowptr,
                          const int *colind, int nbr, const void *mask) {
   for (int I = 0; I < nbr; ++I) {
-    uint32_t out = 0;
+    uint64_t out = 0;
     for (int r = 0; r < T; ++r) {
       int cnt = 0;
       for (int k = rowptr[I]; k < rowptr[I + 1]; ++k)
         cnt += popc(get_word(A, T, (long long)k * T + r) & get_word(B, T, colind[k]));
-      if (cnt > 0) out |= 1u << (T - 1 - r);
+      if (cnt > 0) out |= 1ull << (T - 1 - r);
     }
     set_word(C, T, I, out & ~get_word(mask, T, I));
   }

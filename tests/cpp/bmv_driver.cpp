@@ -70,7 +70,8 @@ static int CSR2HSBSF() {
     case 4: bgb_ref::csr2bsr_batch_4(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned char *)tA, tiledim, nblockrows, nblocks); break;
     case 8: bgb_ref::csr2bsr_batch_8(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned char *)tA, tiledim, nblockrows, nblocks); break;
     case 16: bgb_ref::csr2bsr_batch_16(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned short *)tA, tiledim, nblockrows, nblocks); break;
-    default: bgb_ref::csr2bsr_batch_32(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned *)tA, tiledim, nblockrows, nblocks); break;
+    case 32: bgb_ref::csr2bsr_batch_32(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned *)tA, tiledim, nblockrows, nblocks); break;
+    default: bgb_ref::csr2bsr_batch_64(h_csrRowPtr.data(), h_csrColInd.data(), nrows, ncols, nnz, bsrRowPtr, bsrColInd, (unsigned long long *)tA, tiledim, nblockrows, nblocks); break;
   }
   return bgb_ref::last_status() ? fail("csr2bsr_batch") : 0;
 }
@@ -92,7 +93,8 @@ static int packRandVec() {
     case 4: bgb_ref::ToBit4Row(fB, (unsigned char *)tB, nblockrows); break;
     case 8: bgb_ref::ToBit8Row(fB, (unsigned char *)tB, nblockrows); break;
     case 16: bgb_ref::ToBit16Row(fB, (unsigned short *)tB, nblockrows); break;
-    default: bgb_ref::ToBit32Row(fB, (unsigned *)tB, nblockrows); break;
+    case 32: bgb_ref::ToBit32Row(fB, (unsigned *)tB, nblockrows); break;
+    default: bgb_ref::ToBit64Row(fB, (unsigned long long *)tB, nblockrows); break;
   }
   return bgb_ref::last_status() ? fail("ToBitRow") : 0;
 }
@@ -112,21 +114,24 @@ static double evalHSBSFBmv(const std::string &op) {
         case 4: bgb_ref::bmv4_bin_bin_full((unsigned char *)tA, (unsigned char *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 8: bgb_ref::bmv8_bin_bin_full((unsigned char *)tA, (unsigned char *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 16: bgb_ref::bmv16_bin_bin_full((unsigned short *)tA, (unsigned short *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
-        default: bgb_ref::bmv32_bin_bin_full((unsigned *)tA, (unsigned *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        case 32: bgb_ref::bmv32_bin_bin_full((unsigned *)tA, (unsigned *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        default: bgb_ref::bmv64_bin_bin_full((unsigned long long *)tA, (unsigned long long *)tB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
       }
     } else if (op == "bin_full_full") {
       switch (tiledim) {
         case 4: bgb_ref::bmv4_bin_full_full((unsigned char *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 8: bgb_ref::bmv8_bin_full_full((unsigned char *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 16: bgb_ref::bmv16_bin_full_full((unsigned short *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
-        default: bgb_ref::bmv32_bin_full_full((unsigned *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        case 32: bgb_ref::bmv32_bin_full_full((unsigned *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        default: bgb_ref::bmv64_bin_full_full((unsigned long long *)tA, fB, fC, bsrRowPtr, bsrColInd, nblockrows); break;
       }
     } else {
       switch (tiledim) {
         case 4: bgb_ref::bmv4_bin_bin_bin((unsigned char *)tA, (unsigned char *)tB, (unsigned char *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 8: bgb_ref::bmv8_bin_bin_bin((unsigned char *)tA, (unsigned char *)tB, (unsigned char *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
         case 16: bgb_ref::bmv16_bin_bin_bin((unsigned short *)tA, (unsigned short *)tB, (unsigned short *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
-        default: bgb_ref::bmv32_bin_bin_bin((unsigned *)tA, (unsigned *)tB, (unsigned *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        case 32: bgb_ref::bmv32_bin_bin_bin((unsigned *)tA, (unsigned *)tB, (unsigned *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
+        default: bgb_ref::bmv64_bin_bin_bin((unsigned long long *)tA, (unsigned long long *)tB, (unsigned long long *)fC, bsrRowPtr, bsrColInd, nblockrows); break;
       }
     }
     if (bgb_ref::last_status()) return -1.0;
@@ -163,12 +168,12 @@ static bool verifyResult(const std::string &op) {
   std::vector<unsigned char> got((size_t)nblockrows * wb);
   hipMemcpy(got.data(), fC, got.size(), hipMemcpyDeviceToHost);
   for (int I = 0; I < nblockrows; ++I) {
-    uint32_t w = 0;
+    uint64_t w = 0;
     memcpy(&w, &got[(size_t)I * wb], wb);
     for (int r = 0; r < tiledim; ++r) {
       const int v = I * tiledim + r;
       const bool want = v < nrows && cnt[v] > 0;
-      if ((((w >> (tiledim - 1 - r)) & 1u) != 0) != want) return false;
+      if ((((w >> (tiledim - 1 - r)) & 1ull) != 0) != want) return false;
     }
   }
   return true;

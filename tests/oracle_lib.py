@@ -12,7 +12,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
 REF_SO = os.path.join(REPO, "oracle", "_ref", "libreadmtx_ref.so")
 
-WORD_NP = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32}
+WORD_NP = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
 
 
 def _p(a):

@@ -14,7 +14,7 @@ DRIVER = os.path.join(os.path.dirname(__file__), "cpp", "bmv_driver")
 
 
 @pytest.mark.parametrize("op", ["bin_bin_full", "bin_bin_bin", "bin_full_full"])
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [4, 8, 16, 32, 64])
 def test_reference_shaped_driver(op, T):
     assert os.path.exists(DRIVER), "driver not built (run __graft_entry__.build())"
     for graph in ("er:4096:16", "rmat:14:16", "grid:200:150"):

@@ -94,7 +94,7 @@ def pr_errors(pr, ref):
     return np.max(np.abs(pr.astype(np.float64) - ref) / ref)
 
 
-@pytest.mark.parametrize("T", TS)
+@pytest.mark.parametrize("T", TS + (64,))
 def test_pagerank_grid_modes(orc, dev, T):
     from bgb import b2sr as B
     from bgb import graph

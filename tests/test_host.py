@@ -37,7 +37,7 @@ def test_info_calls():
     L = bgb.lib()
     assert L.bgb_version() >= 100
     assert L.bgb_status_string(0) == b"ok"
-    assert [L.bgb_word_bytes(t) for t in (4, 8, 16, 32, 5)] == [1, 1, 2, 4, 0]
+    assert [L.bgb_word_bytes(t) for t in (4, 8, 16, 32, 64, 5)] == [1, 1, 2, 4, 8, 0]
     assert L.bgb_packed_vec_words(32, 128) == 128
     assert L.bgb_packed_vec_words(4, 5) == 8
     assert L.bgb_packed_vec_words(3, 5) == -1

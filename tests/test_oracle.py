@@ -14,7 +14,7 @@ import pytest
 from oracle_lib import REF_SO, WORD_NP, RefReadMtx
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "b2sr_5x5.json")
-TS = (4, 8, 16, 32)
+TS = (4, 8, 16, 32, 64)
 
 
 @pytest.fixture(scope="module")
@@ -64,7 +64,7 @@ def test_golden_b2sr_layout(orc, gold):
         brp, bci, val = orc.b2sr(T, n, n, rp, ci)
         assert brp.tolist() == g["rowptr"], T
         assert bci.tolist() == g["colind"], T
-        assert val.astype(np.int64).tolist() == g["val"], T
+        assert val.tolist() == g["val"], T
 
 
 def test_golden_vector_and_ops(orc, gold):
@@ -80,15 +80,15 @@ def test_golden_vector_and_ops(orc, gold):
         nbr = -(-n // T)
         brp, bci, val = orc.b2sr(T, n, n, rp, ci)
         xb = orc.pack_vec(T, xbits, nbr)
-        assert xb.astype(np.int64).tolist() == g["xb"], T
+        assert xb.tolist() == g["xb"], T
         assert orc.b2sr_bin_bin_full(T, val, xb, brp, bci).tolist() == g["bin_bin_full"]
-        assert orc.b2sr_bin_bin_bin(T, val, xb, brp, bci).astype(np.int64).tolist() == g["bin_bin_bin"]
+        assert orc.b2sr_bin_bin_bin(T, val, xb, brp, bci).tolist() == g["bin_bin_bin"]
         mask = np.array(g["mask"], dtype=np.uint64).astype(WORD_NP[T])
-        assert orc.b2sr_masked(T, val, xb, brp, bci, mask).astype(np.int64).tolist() == g["masked"]
+        assert orc.b2sr_masked(T, val, xb, brp, bci, mask).tolist() == g["masked"]
         # the mask is the packed form of the visited rows
         mbits = np.zeros(n, np.uint8)
         mbits[gold["mask_rows"]] = 1
-        assert orc.pack_u8(T, mbits, nbr).astype(np.int64).tolist() == g["mask"]
+        assert orc.pack_u8(T, mbits, nbr).tolist() == g["mask"]
         xpad = np.zeros(nbr * T, np.float32)
         xpad[:n] = xf
         ff = orc.b2sr_bin_full_full(T, val, xpad, brp, bci)
