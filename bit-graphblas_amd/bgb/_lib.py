@@ -39,6 +39,10 @@ SIGNATURES = {
     "bgb_b2sr_nnzb": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "bgb_csr2b2sr": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
     "bgb_pack_vec": (_c_int, [_c_int, _vp, _c_int, _vp, _vp]),
+    "bgb_csr2csc": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bgb_b2sr_transpose": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bgb_tile_row_stats": (_c_int, [_c_int, _vp, _vp, _c_int, _vp, _vp, _vp]),
+    "bgb_estimate_tiles": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_set_run_length": (_c_int, [_c_int, _c_int]),
     "bgb_get_run_length": (_c_int, [_c_int]),
     "bgb_set_x_staging": (_c_int, [_c_int, _c_int]),

@@ -81,6 +81,44 @@ class B2SR:
         v = self.val[: self.nblocks * self.T].cpu().numpy().view(WORD_NUMPY[self.T])
         return rp, ci, v
 
+    def transpose(self):
+        """The B2SR of A^T, built on the GPU from this one (bgb_b2sr_transpose)."""
+        T, dev = self.T, self.device
+        nbc = (self.ncols + T - 1) // T
+        rp = torch.empty(nbc + 1, dtype=torch.int32, device=dev)
+        colind = torch.empty(max(self.nblocks, 1) + 4, dtype=torch.int32, device=dev)
+        val = torch.zeros(max(self.nblocks, 1) * T + 16, dtype=WORD_TORCH[T], device=dev)
+        call("bgb_b2sr_transpose", T, self.nblockrows, nbc, self.nblocks, _ptr(self.rowptr), _ptr(self.colind),
+             _ptr(self.val), _ptr(rp), _ptr(colind), _ptr(val), _stream())
+        return B2SR(T, self.ncols, self.nrows, rp, colind, val, self.nblocks)
+
+    def row_stats(self):
+        """printMatrixStats_T: (tile_per_row, nnz_tile_per_row), int32 [NBR*T] each."""
+        n = self.nblockrows * self.T
+        tpr = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        nzr = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        call("bgb_tile_row_stats", self.T, _ptr(self.val), _ptr(self.rowptr), self.nblockrows, _ptr(tpr),
+             _ptr(nzr), _stream())
+        return tpr[:n], nzr[:n]
+
+
+def csr2csc(m, n, nnz, d_rowptr, d_colind, d_val=None):
+    """cusparseScsr2csc on the GPU: (colptr[n+1], rowind[nnz], val[nnz] or None)."""
+    dev = d_rowptr.device
+    colptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    rowind = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    cval = None if d_val is None else torch.empty(max(nnz, 1), dtype=torch.float32, device=dev)
+    call("bgb_csr2csc", m, n, nnz, _ptr(d_val), _ptr(d_rowptr), _ptr(d_colind), _ptr(cval), _ptr(rowind),
+         _ptr(colptr), _stream())
+    return colptr, rowind[:nnz], (None if cval is None else cval[:nnz])
+
+
+def estimate_tiles(T, nrows, d_rowptr, d_colind):
+    """estimatevec on the GPU: runs of equal colind / T summed over rows."""
+    out = torch.empty(1, dtype=torch.int64, device=d_rowptr.device)
+    call("bgb_estimate_tiles", T, nrows, _ptr(d_rowptr), _ptr(d_colind), _ptr(out), _stream())
+    return int(out.item())
+
 
 def pack_vec(x, T, nblockrows=None):
     """ToBit{T}Row: float vector -> packed words (length ceil(NBR/T)*T)."""

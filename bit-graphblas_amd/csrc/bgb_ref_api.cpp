@@ -74,6 +74,11 @@ void csr2bsr_nnz(const int *csrRowPtr, const int *csrColInd, int nrows, int ncol
   record(bgb_b2sr_nnzb(blocksize, nrows, ncols, nnz, csrRowPtr, csrColInd, bsrRowPtr, nblocks, nullptr));
 }
 
+void csr2csc(int m, int n, int nnz, const float *csrVal, const int *csrRowPtr, const int *csrColInd,
+             float *cscVal, int *cscRowInd, int *cscColPtr) {
+  record(bgb_csr2csc(m, n, nnz, csrVal, csrRowPtr, csrColInd, cscVal, cscRowInd, cscColPtr, nullptr));
+}
+
 void csr2bsr_batch_4(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz, int *bsrRowPtr,
                      int *bsrColInd, uchar *bsrVal, int blocksize, int, int nblocks) {
   batch(4, h_csrRowPtr, h_csrColInd, nrows, ncols, nnz, bsrRowPtr, bsrColInd, bsrVal, blocksize, nblocks);

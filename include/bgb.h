@@ -82,6 +82,32 @@ int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
 int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
+ * Transposes (GPU). The reference transposes a directed graph's CSR with   *
+ * cusparseScsr2csc before packing it (transposeMtx,                        *
+ * test_bin_bin_bin.cu:110-136). Both calls synchronise `stream` once (after *
+ * the structure check) and are deterministic.                              *
+ * ------------------------------------------------------------------------ */
+
+/* CSR (m x n) -> CSC, i.e. the CSR of the transpose: d_cscColPtr[n+1],
+ * d_cscRowInd[nnz] (rows ascending inside each column), and d_cscVal[nnz]
+ * when values are given (d_csrVal and d_cscVal both NULL = structure only).
+ * Argument order of cusparseScsr2csc (CUSPARSE_ACTION_NUMERIC,
+ * CUSPARSE_INDEX_BASE_ZERO). Rows need not be sorted; BGB_ERR_FORMAT if
+ * rowptr is not monotone from 0 to nnz or an index is outside [0, n). */
+int bgb_csr2csc(int m, int n, int nnz, const float *d_csrVal, const int *d_csrRowPtr,
+                const int *d_csrColInd, float *d_cscVal, int *d_cscRowInd, int *d_cscColPtr,
+                bgb_stream_t stream);
+
+/* B2SR of A^T from the B2SR of A (no reference counterpart): block (I, J)
+ * becomes block (J, I) with its T x T bit tile transposed. nblockcols =
+ * ceil(ncols / T) of A = the block rows of A^T; outputs d_t_rowptr
+ * [nblockcols+1], d_t_colind[nblocks], d_t_val[nblocks*T] -- identical to
+ * packing the CSR of A^T with bgb_csr2b2sr. */
+int bgb_b2sr_transpose(int T, int nblockrows, int nblockcols, int nblocks, const int *d_rowptr,
+                       const int *d_colind, const void *d_val, int *d_t_rowptr,
+                       int *d_t_colind, void *d_t_val, bgb_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
  * BMV (GPU). A = bsrVal, rowptr/colind = bsrRowPtr/bsrColInd, nblocks =     *
  * rowptr[nblockrows] (passed so that no device read is needed).             *
  * accumulate = 1 reproduces the reference's in-place accumulation           *
@@ -231,6 +257,21 @@ int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colin
  * T = 4, 8, 16, 32 (arrays of 4, in that order; h_nonempty4 may be NULL). */
 int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colind,
                    long long *h_nblocks4, long long *h_nonempty4);
+
+/* GPU tile statistics of a B2SR, as the reference's printMatrixStats_T
+ * (utility.cu:808-930), for every T: for row i of block row I,
+ * d_tile_per_row[I*T+i] = tiles in block row I and d_nnz_tile_per_row[I*T+i]
+ * = those tiles whose word i is non-zero (arrays of nblockrows*T ints). */
+int bgb_tile_row_stats(int T, const void *d_A, const int *d_rowptr, int nblockrows,
+                       int *d_tile_per_row, int *d_nnz_tile_per_row, bgb_stream_t stream);
+
+/* The reference's estimatevec (utility.cu:955-975) on a device CSR: *d_out =
+ * sum over rows of the runs of equal colind / T. For rows with ascending
+ * columns this is the number of non-zero row words of the B2SR (the sum of
+ * d_nnz_tile_per_row) and an upper bound on its tiles. *d_out is overwritten
+ * (the reference adds into a caller-zeroed int). */
+int bgb_estimate_tiles(int T, int nrows, const int *d_rowptr, const int *d_colind,
+                       long long *d_out, bgb_stream_t stream);
 
 /* Picks T for an op family (0: bin_bin_*, 1: bin_full_full, 2: K=16 MFMA,
  * 3: a level of the batched BFS)

@@ -31,6 +31,13 @@ int last_status();
 void csr2bsr_nnz(const int *csrRowPtr, const int *csrColInd, int nrows, int ncols, int nnz, int blocksize,
                  int *bsrRowPtr, int *nblocks);
 
+// cusparseScsr2csc(handle, m, n, nnz, csrVal, csrRowPtr, csrColInd, cscVal,
+// cscRowInd, cscColPtr, CUSPARSE_ACTION_NUMERIC, CUSPARSE_INDEX_BASE_ZERO) as
+// transposeMtx calls it (test_bin_bin_bin.cu:110-136); device arrays. csrVal
+// and cscVal may both be NULL (structure only).
+void csr2csc(int m, int n, int nnz, const float *csrVal, const int *csrRowPtr, const int *csrColInd,
+             float *cscVal, int *cscRowInd, int *cscColPtr);
+
 // csr2bsr_batch_{4,8,16,32,64} (csr2bsr_batch_bsrbmv.cu:16,140,263,386,510): host
 // CSR in, device B2SR out; bsrRowPtr already filled by the nnz pass.
 void csr2bsr_batch_4(const int *h_csrRowPtr, const int *h_csrColInd, int nrows, int ncols, int nnz,

@@ -23,6 +23,18 @@
  *                          semirings -- the "CPU CSR computation" the B2SR path
  *                          must match (the reference itself only compares with
  *                          cuSPARSE csrmv, test_bin_bin_bin.cu:284-323,428-466)
+ *   orc_csr2csc            cusparseScsr2csc(..., CUSPARSE_ACTION_NUMERIC,
+ *                          CUSPARSE_INDEX_BASE_ZERO) as used by transposeMtx
+ *                          (test_bin_bin_bin.cu:110-136): counting sort by
+ *                          column, row order kept inside each column
+ *   orc_b2sr_transpose     B2SR of A^T from the B2SR of A: block (I, J) becomes
+ *                          (J, I) with its T x T bit tile transposed (no
+ *                          reference counterpart; equal to packing the CSR of A^T)
+ *   orc_tile_row_stats     printMatrixStats_{4,8,16,32} (utility.cu:808-930):
+ *                          per row of every block row, the block row's tile count
+ *                          and the number of those tiles whose row word is non-zero
+ *   orc_estimate_tiles     estimatevec (utility.cu:955-975): per CSR row, the
+ *                          number of runs of equal colind/T, summed
  *   orc_bfs, orc_pagerank_f64, orc_csr_spmm_f64
  *                          build-only rows (multi-source BFS, PageRank, K-RHS)
  *                          with no reference counterpart: plain queue BFS, FP64
@@ -342,6 +354,85 @@ void orc_pagerank_f64(int n, const int *rowptr, const int *colind, double d, int
   }
   free(y);
   free(nx);
+}
+
+/* ---------------- transposes ---------------- */
+/* cusparseScsr2csc semantics (zero base, numeric): cscColPtr[j] = entries with
+ * column < j; entries of column j in ascending CSR position (= ascending row).
+ * vals / cscVal may be NULL (structure only). */
+void orc_csr2csc(int nrows, int ncols, long long nnz, const float *vals, const int *rowptr,
+                 const int *colind, float *cscVal, int *cscRowInd, int *cscColPtr) {
+  memset(cscColPtr, 0, sizeof(int) * ((size_t)ncols + 1));
+  for (long long e = 0; e < nnz; ++e) cscColPtr[colind[e] + 1]++;
+  for (int j = 0; j < ncols; ++j) cscColPtr[j + 1] += cscColPtr[j];
+  int *next = (int *)malloc(sizeof(int) * ((size_t)ncols + 1));
+  memcpy(next, cscColPtr, sizeof(int) * ((size_t)ncols + 1));
+  for (int r = 0; r < nrows; ++r)
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      const int p = next[colind[e]]++;
+      cscRowInd[p] = r;
+      if (vals && cscVal) cscVal[p] = vals[e];
+    }
+  free(next);
+}
+
+/* Transposed B2SR. nbc = block columns of A (= block rows of A^T). Bit c of
+ * word r of tile (I, J) is A(I*T + r, J*T + c) (bit T-1-c); the transposed tile
+ * (J, I) holds it as bit T-1-r of word c. */
+void orc_b2sr_transpose(int T, int nbr, int nbc, const int *rowptr, const int *colind, const void *val,
+                        int *t_rowptr, int *t_colind, void *t_val) {
+  const long long nblocks = rowptr[nbr];
+  memset(t_rowptr, 0, sizeof(int) * ((size_t)nbc + 1));
+  for (long long k = 0; k < nblocks; ++k) t_rowptr[colind[k] + 1]++;
+  for (int J = 0; J < nbc; ++J) t_rowptr[J + 1] += t_rowptr[J];
+  int *next = (int *)malloc(sizeof(int) * ((size_t)nbc + 1));
+  memcpy(next, t_rowptr, sizeof(int) * ((size_t)nbc + 1));
+  for (int I = 0; I < nbr; ++I)
+    for (int k = rowptr[I]; k < rowptr[I + 1]; ++k) {
+      const int p = next[colind[k]]++;
+      t_colind[p] = I;
+      for (int c = 0; c < T; ++c) {
+        uint64_t w = 0;
+        for (int r = 0; r < T; ++r)
+          if ((get_word(val, T, (long long)k * T + r) >> (T - 1 - c)) & 1) w |= 1ull << (T - 1 - r);
+        set_word(t_val, T, (long long)p * T + c, w);
+      }
+    }
+  free(next);
+}
+
+/* ---------------- tile statistics (utility.cu:808-975) ---------------- */
+/* printMatrixStats_T: for row i of block row I, tile_per_row[I*T+i] = tiles in
+ * block row I, nnz_tile_per_row[I*T+i] = those tiles whose word i is non-zero. */
+void orc_tile_row_stats(int T, int nbr, const int *rowptr, const void *val, int *tile_per_row,
+                        int *nnz_tile_per_row) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int I = 0; I < nbr; ++I) {
+    const int load = rowptr[I + 1] - rowptr[I];
+    for (int i = 0; i < T; ++i) {
+      int nz = 0;
+      for (int j = rowptr[I]; j < rowptr[I + 1]; ++j) nz += get_word(val, T, (long long)j * T + i) != 0;
+      tile_per_row[(long long)I * T + i] = load;
+      nnz_tile_per_row[(long long)I * T + i] = nz;
+    }
+  }
+}
+
+/* estimatevec: sum over CSR rows of the number of runs of equal colind / T (an
+ * upper bound on the tiles: a block column is counted once per row). */
+long long orc_estimate_tiles(int nrows, const int *rowptr, const int *colind, int T) {
+  long long total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 1024)
+  for (int r = 0; r < nrows; ++r) {
+    int count = 0, last = -1;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      const int J = colind[e] / T;
+      if (e == rowptr[r] || J != last) ++count;
+      last = J;
+    }
+    total += count;
+  }
+  return total;
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

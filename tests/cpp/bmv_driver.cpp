@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bgb.h"
@@ -43,6 +44,7 @@ static int readGraphCSR(const char *spec) {
   if (sscanf(spec, "rmat:%d:%d", &a, &b) == 2) st = bgb_gen_rmat(a, b, 0.57, 0.19, 0.19, 1, &g);
   else if (sscanf(spec, "er:%d:%d", &a, &b) == 2) st = bgb_gen_er(a, (long long)a * b / 2, 1, &g);
   else if (sscanf(spec, "grid:%d:%d", &a, &b) == 2) st = bgb_gen_grid2d(a, b, &g);
+  else if (strncmp(spec, "mtx:", 4) == 0) st = bgb_read_mtx(spec + 4, 0, &g);  // readMtx(..., directed = 0)
   else return 2;
   if (st) return fail("generator");
   nrows = bgb_csr_nrows(g);
@@ -55,6 +57,25 @@ static int readGraphCSR(const char *spec) {
   hipMalloc(&csrColInd, sizeof(int) * (nnz > 0 ? nnz : 1));
   hipMemcpy(csrRowPtr, h_csrRowPtr.data(), sizeof(int) * (nrows + 1), hipMemcpyHostToDevice);
   hipMemcpy(csrColInd, h_csrColInd.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+  return 0;
+}
+
+// transposeMtx (test_bin_bin_bin.cu:110-136): csr2csc on the device, then the
+// CSC arrays serve as the CSR (of A^T).
+static int transposeMtx() {
+  int *cscRowInd = nullptr, *cscColPtr = nullptr;
+  hipMalloc(&cscRowInd, sizeof(int) * (nnz > 0 ? nnz : 1));
+  hipMalloc(&cscColPtr, sizeof(int) * (ncols + 1));
+  bgb_ref::csr2csc(nrows, ncols, nnz, nullptr, csrRowPtr, csrColInd, nullptr, cscRowInd, cscColPtr);
+  if (bgb_ref::last_status()) return fail("csr2csc");
+  std::swap(nrows, ncols);
+  hipFree(csrRowPtr);
+  hipFree(csrColInd);
+  csrRowPtr = cscColPtr;
+  csrColInd = cscRowInd;
+  h_csrRowPtr.resize(nrows + 1);
+  hipMemcpy(h_csrRowPtr.data(), csrRowPtr, sizeof(int) * (nrows + 1), hipMemcpyDeviceToHost);
+  hipMemcpy(h_csrColInd.data(), csrColInd, sizeof(int) * nnz, hipMemcpyDeviceToHost);
   return 0;
 }
 
@@ -181,13 +202,16 @@ static bool verifyResult(const std::string &op) {
 
 int main(int argc, char **argv) {
   if (argc < 4) {
-    fprintf(stderr, "usage: %s <op> <T> <graph>\n", argv[0]);
+    fprintf(stderr, "usage: %s <op> <T> <graph: rmat:S:EF | er:N:DEG | grid:W:H | mtx:PATH> [transpose]\n",
+            argv[0]);
     return 2;
   }
   const std::string op = argv[1];
   tiledim = atoi(argv[2]);
+  const int transpose = argc > 4 ? atoi(argv[4]) : 0;  // the reference drivers' argv[2]
   hipSetDevice(0);
   if (readGraphCSR(argv[3])) return 1;
+  if (transpose && transposeMtx()) return 1;
   genRandVec(op != "bin_full_full");
   if (CSR2HSBSF() || packRandVec()) return 1;
   const double ms = evalHSBSFBmv(op);

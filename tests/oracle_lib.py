@@ -42,6 +42,10 @@ class Oracle:
             "orc_csr_spmm_f64": (None, [ci, vp, vp, ci, vp, vp]),
             "orc_bfs": (ci, [ci, vp, vp, ci, vp]),
             "orc_pagerank_f64": (None, [ci, vp, vp, dbl, ci, vp, vp]),
+            "orc_csr2csc": (None, [ci, ci, ll, vp, vp, vp, vp, vp, vp]),
+            "orc_b2sr_transpose": (None, [ci, ci, ci, vp, vp, vp, vp, vp, vp]),
+            "orc_tile_row_stats": (None, [ci, ci, vp, vp, vp, vp]),
+            "orc_estimate_tiles": (ll, [ci, vp, vp, ci]),
             "orc_num_threads": (ci, []),
             "orc_set_threads": (None, [ci]),
         }
@@ -163,6 +167,44 @@ class Oracle:
         res = np.zeros(max(iters, 1), np.float64)
         self.L.orc_pagerank_f64(n, _p(rowptr), _p(colind), damping, iters, _p(pr), _p(res))
         return pr, res[:iters]
+
+    # -- transposes / statistics -------------------------------------------
+    def csr2csc(self, nrows, ncols, rowptr, colind, vals=None):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        nnz = int(rowptr[-1])
+        colind = np.ascontiguousarray(colind if nnz else np.zeros(1), np.int32)
+        vals = None if vals is None else np.ascontiguousarray(vals, np.float32)
+        cp = np.zeros(ncols + 1, np.int32)
+        ri = np.zeros(max(nnz, 1), np.int32)
+        cv = None if vals is None else np.zeros(max(nnz, 1), np.float32)
+        self.L.orc_csr2csc(nrows, ncols, nnz, _p(vals), _p(rowptr), _p(colind), _p(cv), _p(ri), _p(cp))
+        return cp, ri[:nnz], (None if cv is None else cv[:nnz])
+
+    def b2sr_transpose(self, T, nbc, rowptr, colind, val):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        nb = int(rowptr[-1])
+        colind = np.ascontiguousarray(colind if nb else np.zeros(1), np.int32)
+        val = np.ascontiguousarray(val if nb else np.zeros(T), WORD_NP[T])
+        trp = np.zeros(nbc + 1, np.int32)
+        tci = np.zeros(max(nb, 1), np.int32)
+        tval = np.zeros(max(nb, 1) * T, WORD_NP[T])
+        self.L.orc_b2sr_transpose(T, rowptr.size - 1, nbc, _p(rowptr), _p(colind), _p(val), _p(trp),
+                                  _p(tci), _p(tval))
+        return trp, tci[:nb], tval[: nb * T]
+
+    def tile_row_stats(self, T, rowptr, val):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        nbr = rowptr.size - 1
+        val = np.ascontiguousarray(val if rowptr[-1] else np.zeros(T), WORD_NP[T])
+        tpr = np.zeros(max(nbr * T, 1), np.int32)
+        nzr = np.zeros(max(nbr * T, 1), np.int32)
+        self.L.orc_tile_row_stats(T, nbr, _p(rowptr), _p(val), _p(tpr), _p(nzr))
+        return tpr[: nbr * T], nzr[: nbr * T]
+
+    def estimate_tiles(self, rowptr, colind, T):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        colind = np.ascontiguousarray(colind if len(colind) else np.zeros(1), np.int32)
+        return int(self.L.orc_estimate_tiles(rowptr.size - 1, _p(rowptr), _p(colind), T))
 
     def threads(self):
         return self.L.orc_num_threads()

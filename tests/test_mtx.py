@@ -112,3 +112,26 @@ def test_read_mtx_missing_file():
     import ctypes
     h = ctypes.c_void_p()
     assert bgb.lib().bgb_read_mtx(b"/nonexistent/x.mtx", 0, ctypes.byref(h)) == BGB_ERR_ARG
+
+
+def test_mycielskian_inputs(tmp_path):
+    """The sweep's Mycielski inputs: known sizes (M4 is the Groetzsch graph,
+    11 vertices and 20 edges), the size recurrence, and a symmetric read."""
+    from mtx_files import mycielskian, write_mycielskian
+    sizes = {k: (mycielskian(k)[0], len(mycielskian(k)[1])) for k in range(2, 9)}
+    assert sizes[2] == (2, 1) and sizes[3] == (5, 5) and sizes[4] == (11, 20) and sizes[5] == (23, 71)
+    for k in range(2, 8):
+        n, e = sizes[k]
+        assert sizes[k + 1] == (2 * n + 1, 3 * e + n)
+    # triangle-free: no edge (a, b) with a common neighbour
+    n, e = mycielskian(6)
+    adj = np.zeros((n, n), bool)
+    adj[e[:, 0], e[:, 1]] = adj[e[:, 1], e[:, 0]] = True
+    assert not (adj & (adj.astype(np.int64) @ adj.astype(np.int64) > 0)).any()
+    p = str(tmp_path / "mycielskian6.mtx")
+    write_mycielskian(p, 6)
+    csr = graph.read_mtx(p)
+    assert csr.nrows == n and csr.nnz == 2 * len(e)
+    D = np.zeros((n, n), bool)
+    D[np.repeat(np.arange(n), np.diff(csr.rowptr)), csr.colind] = True
+    assert (D == adj).all()

@@ -259,3 +259,70 @@ def test_pagerank_vs_dense_power_iteration(orc):
     pr, _ = orc.pagerank(rp, ci, 0.85, 30)
     np.testing.assert_allclose(pr, p, rtol=1e-12)
     assert abs(pr.sum() - 1) < 1e-12
+
+
+# -- transposes and tile statistics -------------------------------------------
+@pytest.mark.parametrize("shape", [(1, 1), (6, 9), (40, 33), (130, 70)])
+def test_csr2csc_vs_dense_transpose(orc, shape):
+    rng = np.random.default_rng(sum(shape))
+    D, rp, ci = rand_csr(rng, *shape, 0.2)
+    vals = rng.standard_normal(ci.size).astype(np.float32)
+    W = np.zeros(shape, np.float32)
+    W[np.repeat(np.arange(shape[0]), np.diff(rp)), ci] = vals
+    cp, ri, cv = orc.csr2csc(shape[0], shape[1], rp, ci, vals)
+    Dt = D.T
+    assert cp.tolist() == [0] + np.cumsum(Dt.sum(1)).tolist()
+    assert ri.tolist() == np.nonzero(Dt)[1].tolist()
+    assert cv.tolist() == W.T[np.nonzero(Dt)].tolist()
+
+
+def test_csr2csc_hand_case(orc):
+    # 3 x 4: rows {1, 3}, {}, {0, 1}  ->  columns {2}, {0, 2}, {}, {0}
+    cp, ri, _ = orc.csr2csc(3, 4, [0, 2, 2, 4], [1, 3, 0, 1])
+    assert cp.tolist() == [0, 1, 3, 3, 4]
+    assert ri.tolist() == [2, 0, 2, 0]
+
+
+@pytest.mark.parametrize("T", TS)
+@pytest.mark.parametrize("shape", [(1, 1), (5, 7), (70, 45), (129, 200)])
+def test_b2sr_transpose_vs_dense(orc, T, shape):
+    rng = np.random.default_rng(T * 1000 + shape[0])
+    D, _, _ = rand_csr(rng, *shape, 0.1)
+    brp, bci, bval = dense_tiles(D, T)
+    nbc = -(-shape[1] // T)
+    trp, tci, tval = orc.b2sr_transpose(T, nbc, brp, bci, bval)
+    erp, eci, ev = dense_tiles(np.ascontiguousarray(D.T), T)
+    assert trp.tolist() == erp.tolist() and tci.tolist() == eci.tolist()
+    assert tval.tolist() == ev.tolist()
+    # transposing twice gives the original back
+    rrp, rci, rval = orc.b2sr_transpose(T, brp.size - 1, trp, tci, tval)
+    assert rrp.tolist() == brp.tolist() and rci.tolist() == bci.tolist() and rval.tolist() == bval.tolist()
+
+
+@pytest.mark.parametrize("T", TS)
+def test_tile_row_stats_vs_dense(orc, T):
+    rng = np.random.default_rng(T)
+    n = 3 * T + 5
+    D, _, _ = rand_csr(rng, n, 2 * n, 0.03)
+    brp, bci, bval = dense_tiles(D, T)
+    tpr, nzr = orc.tile_row_stats(T, brp, bval)
+    nbr = brp.size - 1
+    P = np.zeros((nbr * T, 2 * n + T), np.int64)
+    P[:n, : 2 * n] = D
+    for I in range(nbr):
+        cols = bci[brp[I]:brp[I + 1]]
+        for i in range(T):
+            assert tpr[I * T + i] == cols.size
+            row = P[I * T + i]
+            assert nzr[I * T + i] == sum(int(row[J * T:(J + 1) * T].any()) for J in cols)
+
+
+@pytest.mark.parametrize("T", TS)
+def test_estimate_tiles_vs_numpy(orc, T):
+    rng = np.random.default_rng(7 + T)
+    D, rp, ci = rand_csr(rng, 150, 300, 0.05)
+    want = sum(np.unique(ci[rp[r]:rp[r + 1]] // T).size for r in range(150))
+    assert orc.estimate_tiles(rp, ci, T) == want
+    # an upper bound on the exact tile count (a block column counted once per row)
+    assert want >= dense_tiles(D, T)[1].size
+    assert orc.estimate_tiles([0, 3, 3], [0, 1, 9], 4) == 2  # columns 0, 1 share block 0
