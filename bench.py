@@ -134,6 +134,32 @@ def cpu_baseline(orc, csr, x01, budget_s=8.0):
     return y, t, reps
 
 
+def cpu_model():
+    """Host CPU model name and logical CPU count (the box's, where this runs)."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "nproc": os.cpu_count()}
+
+
+def timed_reps(fn, budget_s):
+    """Mean seconds of fn() over repetitions filling about budget_s (after one warm-up)."""
+    fn()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or reps >= 200:
+            return el / reps, reps
+
+
 def rocsparse_baseline(torch, csr, x01, reps=20):
     """rocSPARSE csrmv (CSR FP32, alpha=1, beta=0, x as 0/1 floats) on the same graph."""
     import ctypes
@@ -519,8 +545,25 @@ def main():
         cpu = {"value": csr.nnz / tc / 1e9, "unit": "GTEPS", "cores": orc.threads(), "kind": "port",
                "sample": f"oracle CSR count loop (OpenMP) over the full graph, {reps} reps, "
                          f"{tc * 1e3:.2f} ms/BMV",
-               "gpu_matches_cpu": bool(match)}
+               "gpu_matches_cpu": bool(match), "host": cpu_model()}
         log(f"[bench] cpu {cpu['value']:.3f} GTEPS on {cpu['cores']} threads, match={match}")
+        # the same CSR loop on one core, and the oracle's B2SR tile loop (all threads) at the best T
+        x8 = x01.astype(np.uint8)
+        nthreads = orc.threads()
+        orc.set_threads(1)
+        t1, r1 = timed_reps(lambda: orc.csr_count(csr.rowptr, csr.colind, x8), 4.0)
+        orc.set_threads(nthreads)
+        cpu["one_core"] = {"value": round(csr.nnz / t1 / 1e9, 4), "unit": "GTEPS", "reps": r1}
+        Ab = B.B2SR.from_csr(csr, best, dev)
+        hrp, hci, hval = Ab.to_host()
+        hxb = orc.pack_vec(best, x01, Ab.nblockrows)
+        del Ab
+        tb, rb = timed_reps(lambda: orc.b2sr_bin_bin_full(best, hval, hxb, hrp, hci), 4.0)
+        yb = orc.b2sr_bin_bin_full(best, hval, hxb, hrp, hci)[: csr.nrows]
+        cpu["b2sr_tile_loop"] = {"value": round(csr.nnz / tb / 1e9, 4), "unit": "GTEPS", "tile_dim": best,
+                                 "cores": nthreads, "reps": rb, "matches": bool(np.array_equal(yb, y))}
+        log(f"[bench] cpu one core {cpu['one_core']['value']} GTEPS; B2SR tile loop T={best} "
+            f"{cpu['b2sr_tile_loop']['value']} GTEPS")
 
     extras = {}
     wanted = [w for w in args.extras.split(",") if w and w != "none"]

@@ -122,12 +122,19 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   int J[S];
   const int n = (int)(pb - pa);
   if (active) load_run<T, S>(b.A, b.colind, pa, n, q, F, J);
+  // frontier-non-empty words of the run's tiles, gathered unconditionally
+  // (block column 0 for missing tiles, then zeroed) so the loads issue together
   uint32_t X[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? (uint32_t) static_cast<const W *>(b.fnz)[J[s]] : 0u;
+  for (int s = 0; s < S; ++s) X[s] = (uint32_t) static_cast<const W *>(b.fnz)[(active && s < n) ? J[s] : 0];
+#pragma unroll
+  for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : 0u;
   RowWindow rw;
   rw.stage(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane);
-  int I = active ? rw.find(pa) : Ia;
+  // the walk, compiled for row ends in the LDS window and in memory
+  rw.dispatch([&](auto st) {
+  constexpr bool ST = decltype(st)::value;
+  int I = active ? rw.find_t<ST>(pa) : Ia;
   const int I0 = I;
 
   bool any = false;
@@ -141,7 +148,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   if (active) load_visited<T>(b, I, q, vr);
   int held = -1;
-  int e = active ? rw.at(I + 1) : INT_MAX;
+  int e = active ? rw.at_t<ST>(I + 1) : INT_MAX;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (s < n) {
@@ -173,12 +180,12 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
         ++I;
-        e = I < b.nbr ? rw.at(I + 1) : INT_MAX;
+        e = I < b.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
         while (e == pos) {
           load_visited<T>(b, I, q, vr);
           bfs_flush<T>(b, I, q, zero, vr, depth, any);
           ++I;
-          e = I < b.nbr ? rw.at(I + 1) : INT_MAX;
+          e = I < b.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
         }
         if (I < b.nbr) load_visited<T>(b, I, q, vr);
       }
@@ -206,6 +213,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   }
   if (__ballot(any) != 0 && lane == 0) b.ctl[1] = 1;
+  });
 }
 
 // Folds the carries of rows that cross chunks into this level's results.

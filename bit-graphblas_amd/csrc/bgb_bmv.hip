@@ -176,13 +176,20 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   const uint32_t(&F)[S][NW] = in.F;
   const int(&J)[S] = in.J;
 
+  // x words of the run's tiles. The gathers are unconditional (block column
+  // 0 stands in for missing tiles, whose word is then zeroed) so that they
+  // issue back to back instead of one predicated region and wait per tile.
   Word<T> X[S];
   if constexpr (!is_full(OP)) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if constexpr (XL) X[s] = (active && s < n) ? lds_xword<T>(xl, J[s]) : (Word<T>)0;
-      else X[s] = (active && s < n) ? (Word<T>) static_cast<const W *>(a.B)[J[s]] : (Word<T>)0;
+      const bool v = active && s < n;
+      const int j = v ? J[s] : 0;
+      if constexpr (XL) X[s] = lds_xword<T>(xl, j);
+      else X[s] = (Word<T>) static_cast<const W *>(a.B)[j];
     }
+#pragma unroll
+    for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : (Word<T>)0;
   }
 
   // FP ops with T <= 8: the whole x segment of each tile (T values) is loaded
@@ -225,7 +232,11 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   // the wave's staged row ends (checked against rowptr first).
   RowWindow rw;
   put_checked(rw, a.rowptr, a.part, win, chunk, a.nchunks, a.nbr, k0, k0 + CH, in.rows, lane, in.wv);
-  int I = active ? rw.find(pa) : rw.Ia;
+  // The walk, compiled twice: row ends read from the LDS window (ST, the
+  // common case) or from rowptr in memory (chunks spanning > kWin rows).
+  rw.dispatch([&](auto st) {
+  constexpr bool ST = decltype(st)::value;
+  int I = active ? rw.find_t<ST>(pa) : rw.Ia;
   const int I0 = I;
 
   Acc acc[R], zero[R];
@@ -240,7 +251,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   Acc hv[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) hv[i] = (Acc)0;
-  int e = active ? rw.at(I + 1) : INT_MAX;
+  int e = active ? rw.at_t<ST>(I + 1) : INT_MAX;
 
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -291,11 +302,11 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
         ++I;
-        e = I < a.nbr ? rw.at(I + 1) : INT_MAX;
+        e = I < a.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
         while (e == pos) {
           flush_row<T, OP>(a, I, q, zero);
           ++I;
-          e = I < a.nbr ? rw.at(I + 1) : INT_MAX;
+          e = I < a.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
         }
       }
     }
@@ -342,6 +353,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
       for (int i = 0; i < R; ++i) cv[i] = sc[i];
     }
   }
+  });
 }
 
 template <int T, int OP, int S>

@@ -352,6 +352,32 @@ struct RowWindow {
   }
   // rowptr[i] for Ia <= i <= Ib + 1
   __device__ __forceinline__ int at(int i) const { return staged ? win[i - Ia] : rowptr[i]; }
+  // The same with the placement known at compile time (ST = staged): the
+  // runtime select above makes the compiler use flat loads, whose waits also
+  // drain every global load in flight; callers branch once, wave-uniformly,
+  // and use these in the two copies of their walk.
+  template <bool ST>
+  __device__ __forceinline__ int at_t(int i) const {
+    if constexpr (ST) return win[i - Ia];
+    else return rowptr[i];
+  }
+  template <bool ST>
+  __device__ __forceinline__ int find_t(long long p) const {
+    int lo = Ia, hi = Ib;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (at_t<ST>(mid) <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+  // Calls f(std::true_type) when the window is staged, f(std::false_type)
+  // otherwise (a wave-uniform branch).
+  template <class F>
+  __device__ __forceinline__ void dispatch(F &&f) const {
+    if (__builtin_amdgcn_readfirstlane((int)staged)) f(std::true_type{});
+    else f(std::false_type{});
+  }
   // Row of tile p (rowptr[I] <= p < rowptr[I+1]) for p inside the chunk.
   __device__ __forceinline__ int find(long long p) const {
     int lo = Ia, hi = Ib;
@@ -386,8 +412,12 @@ __device__ __forceinline__ void put_checked(RowWindow &rw, const int *__restrict
                                             int2 rows, int lane, const int (&wv)[kWin / 64]) {
   const bool last = chunk + 1 >= nchunks;
   rw.put(rowptr, win, rows.x, rows.y, lane, wv);
-  bool ok = rw.at(rows.x) <= k0 && k0 < rw.at(rows.x + 1);
-  if (!last) ok = ok && rw.at(rows.y) <= kend && kend < rw.at(rows.y + 1);
+  bool ok = false;
+  rw.dispatch([&](auto st) {
+    constexpr bool ST = decltype(st)::value;
+    ok = rw.at_t<ST>(rows.x) <= k0 && k0 < rw.at_t<ST>(rows.x + 1);
+    if (!last) ok = ok && rw.at_t<ST>(rows.y) <= kend && kend < rw.at_t<ST>(rows.y + 1);
+  });
   if (!__builtin_amdgcn_readfirstlane((int)ok)) {
     const int2 ab = wave_find_rows2(rowptr, nbr - 1, k0, last ? k0 : kend, lane);
     const int Ia = ab.x, Ib = last ? nbr - 1 : ab.y;

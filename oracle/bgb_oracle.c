@@ -203,6 +203,7 @@ void orc_pack_u8(int T, const uint8_t *bits, int n, long long nwords, void *out)
 /* ---------------- B2SR tile loops (bsrbmv.cu:277-1229) ---------------- */
 void orc_b2sr_bmv_bin_bin_bin(int T, const void *A, const void *B, void *C, const int *rowptr,
                               const int *colind, int nbr) {
+#pragma omp parallel for schedule(dynamic, 64) /* block rows write disjoint outputs */
   for (int I = 0; I < nbr; ++I) {
     uint64_t out = 0;
     for (int r = 0; r < T; ++r) {
@@ -217,6 +218,7 @@ void orc_b2sr_bmv_bin_bin_bin(int T, const void *A, const void *B, void *C, cons
 
 void orc_b2sr_bmv_bin_bin_full(int T, const void *A, const void *B, int *C, const int *rowptr,
                                const int *colind, int nbr) {
+#pragma omp parallel for schedule(dynamic, 64) /* block rows write disjoint outputs */
   for (int I = 0; I < nbr; ++I)
     for (int r = 0; r < T; ++r) {
       int cnt = 0;
@@ -229,6 +231,7 @@ void orc_b2sr_bmv_bin_bin_full(int T, const void *A, const void *B, int *C, cons
 /* FP32, each row summed in ascending (tile, column) order */
 void orc_b2sr_bmv_bin_full_full(int T, const void *A, const float *B, float *C, const int *rowptr,
                                 const int *colind, int nbr) {
+#pragma omp parallel for schedule(dynamic, 64) /* block rows write disjoint outputs */
   for (int I = 0; I < nbr; ++I)
     for (int r = 0; r < T; ++r) {
       float sum = 0.0f;
@@ -245,6 +248,7 @@ void orc_b2sr_bmv_bin_full_full(int T, const void *A, const float *B, float *C, 
  * shift/nibble defects corrected) */
 void orc_b2sr_bmv_masked(int T, const void *A, const void *B, void *C, const int *rowptr,
                          const int *colind, int nbr, const void *mask) {
+#pragma omp parallel for schedule(dynamic, 64) /* block rows write disjoint outputs */
   for (int I = 0; I < nbr; ++I) {
     uint64_t out = 0;
     for (int r = 0; r < T; ++r) {
