@@ -379,8 +379,8 @@ __device__ __forceinline__ int2 load_part_vec(const BmvArgs &a, int c) {
   int cv = c;
   asm volatile("" : "+v"(cv));  // a VGPR address: vector loads
   const int ia = a.part[cv];
-  const int ib = cv + 1 < a.nchunks ? a.part[cv + 1] : 0;
-  return make_int2(ia, ib);
+  const int ib = a.part[cv + 1 < a.nchunks ? cv + 1 : cv];  // unconditional (see RowWindow::load)
+  return make_int2(ia, cv + 1 < a.nchunks ? ib : 0);
 }
 
 // Bit-vector ops with the whole packed x in LDS (it fits for n up to ~1.1M):
@@ -441,15 +441,17 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
   // roles, so no chunk inputs are copied.
   constexpr int CH = (64 / Tile<T>::L) * S;
   const int nfull = a.nblocks / CH;  // the last chunk may be short
-  int2 pnext = c + nw < nfull ? load_part_vec(a, c + nw) : make_int2(0, 0);
+  // Every prefetch is issued unconditionally -- past the wave's last whole
+  // chunk it re-reads an existing one (clamped index, result unused) -- so
+  // that the walk's waits count exactly the loads it needs.
+  int2 pnext = load_part_vec(a, c + nw < nfull ? c + nw : c);
   auto step = [&](ChunkIn<T, S> &cur, ChunkIn<T, S> &nxt) {
     const int chunk = __builtin_amdgcn_readfirstlane(c);
     const int cn = c + nw;
     const bool more = cn < nfull;  // wave-uniform
-    if (more) {
-      load_chunk<T, S, true>(a, cn, lane, rows_of(cn, pnext), nxt);
-      if (cn + nw < nfull) pnext = load_part_vec(a, cn + nw);
-    }
+    const int cl = more ? cn : chunk;
+    load_chunk<T, S, true>(a, cl, lane, rows_of(cl, pnext), nxt);
+    pnext = load_part_vec(a, cn + nw < nfull ? cn + nw : chunk);
     bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, cur);
     c = cn;
     return more;

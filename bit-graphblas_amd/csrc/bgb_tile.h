@@ -311,15 +311,18 @@ struct RowWindow {
 
   // Entries rowptr[ia .. ib+1] into registers, kWin/64 per lane (lane + 64j);
   // issued early by the pipelined kernels. `ia`, `ib` are wave-uniform.
+  // The loads are unconditional (indices clamped into rowptr[ia .. ib+1],
+  // entries past the window unused): a load inside a branch leaves the
+  // compiler unsure how many loads are in flight, and its next wait then
+  // drains all of them, prefetches included.
   __device__ __forceinline__ static void load(const int *__restrict__ rp, int ia, int ib, int lane,
                                               int (&wv)[kWin / 64]) {
     const int m = ib + 2 - ia;
-    if (m <= kWin) {
+    const int mm = m < kWin ? m : kWin;
 #pragma unroll
-      for (int j = 0; j < kWin / 64; ++j) {
-        const int i = lane + 64 * j;
-        wv[j] = i < m ? rp[ia + i] : 0;
-      }
+    for (int j = 0; j < kWin / 64; ++j) {
+      const int i = lane + 64 * j;
+      wv[j] = rp[ia + (i < mm ? i : mm - 1)];
     }
   }
   // Makes the window of rows [ia, ib] current, from registers filled by load().
