@@ -103,7 +103,7 @@ template <int T, int S>
 __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
-  __shared__ int s_win[kBfsWaves][kWin];
+  __shared__ int s_win[kBfsWaves][kWinAlloc];
   if (b.ctl[2]) return;  // BFS already finished: this captured level is a no-op
   const int depth = b.ctl[0];
   const int lane = threadIdx.x & 63;
@@ -130,11 +130,16 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
   for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : 0u;
   RowWindow rw;
-  rw.stage(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane);
+  int wv[kWin / 64];
+  RowWindow::load(rowptr, Ia, Ib, lane, wv);
+  rw.put(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane, wv);
   // the walk, compiled for row ends in the LDS window and in memory
   rw.dispatch([&](auto st) {
   constexpr bool ST = decltype(st)::value;
-  int I = active ? rw.find_t<ST>(pa) : Ia;
+  int I;
+  if constexpr (ST) I = window_group_rows<L, S>(rw, k0, lane, wv);
+  else I = rw.find_t<false>(pa);
+  I = active ? I : Ia;
   const int I0 = I;
 
   bool any = false;
@@ -148,7 +153,12 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   if (active) load_visited<T>(b, I, q, vr);
   int held = -1;
-  int e = active ? rw.at_t<ST>(I + 1) : INT_MAX;
+  // end of row I and, read one row ahead, of row I+1 (indices clamped into the
+  // window / rowptr; clamped ends belong to rows past the chunk)
+  const int hi = ST ? rw.Ib + 1 : b.nbr;
+  auto row_end = [&](int r) { return r < b.nbr ? rw.at_t<ST>(r + 1 < hi ? r + 1 : hi) : INT_MAX; };
+  int e = active ? row_end(I) : INT_MAX;
+  int e2 = active ? row_end(I + 1) : INT_MAX;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (s < n) {
@@ -180,12 +190,14 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
         ++I;
-        e = I < b.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
+        e = e2;
+        e2 = row_end(I + 1);
         while (e == pos) {
           load_visited<T>(b, I, q, vr);
           bfs_flush<T>(b, I, q, zero, vr, depth, any);
           ++I;
-          e = I < b.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
+          e = e2;
+          e2 = row_end(I + 1);
         }
         if (I < b.nbr) load_visited<T>(b, I, q, vr);
       }

@@ -231,12 +231,21 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   // Block row holding tile pa: rowptr[I] <= pa < rowptr[I+1], searched in
   // the wave's staged row ends (checked against rowptr first).
   RowWindow rw;
-  put_checked(rw, a.rowptr, a.part, win, chunk, a.nchunks, a.nbr, k0, k0 + CH, in.rows, lane, in.wv);
+  const bool restaged = __builtin_amdgcn_readfirstlane(
+      (int)put_checked(rw, a.rowptr, a.part, win, chunk, a.nchunks, a.nbr, k0, k0 + CH, in.rows, lane, in.wv));
   // The walk, compiled twice: row ends read from the LDS window (ST, the
   // common case) or from rowptr in memory (chunks spanning > kWin rows).
   rw.dispatch([&](auto st) {
   constexpr bool ST = decltype(st)::value;
-  int I = active ? rw.find_t<ST>(pa) : rw.Ia;
+  int I;
+  if constexpr (ST) {
+    // the registers hold the window unless put_checked restaged it
+    if (!restaged) I = window_group_rows<L, S>(rw, k0, lane, in.wv);
+    else I = rw.find_t<true>(pa);
+  } else {
+    I = rw.find_t<false>(pa);
+  }
+  I = active ? I : rw.Ia;
   const int I0 = I;
 
   Acc acc[R], zero[R];
@@ -251,7 +260,14 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   Acc hv[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) hv[i] = (Acc)0;
-  int e = active ? rw.at_t<ST>(I + 1) : INT_MAX;
+  // e = end of row I, e2 = end of row I+1, read one row ahead so that a row
+  // end does not wait on the LDS (or memory) read of the next one. Indices
+  // past the window / rowptr are clamped: such rows lie beyond the chunk and
+  // their ends are never compared.
+  const int hi = ST ? rw.Ib + 1 : a.nbr;
+  auto row_end = [&](int r) { return r < a.nbr ? rw.at_t<ST>(r + 1 < hi ? r + 1 : hi) : INT_MAX; };
+  int e = active ? row_end(I) : INT_MAX;
+  int e2 = active ? row_end(I + 1) : INT_MAX;
 
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -302,11 +318,13 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
         ++I;
-        e = I < a.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
-        while (e == pos) {
+        e = e2;
+        e2 = row_end(I + 1);
+        while (e == pos) {  // empty rows
           flush_row<T, OP>(a, I, q, zero);
           ++I;
-          e = I < a.nbr ? rw.at_t<ST>(I + 1) : INT_MAX;
+          e = e2;
+          e2 = row_end(I + 1);
         }
       }
     }
@@ -359,7 +377,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
 template <int T, int OP, int S>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   constexpr int CH = (64 / Tile<T>::L) * S;
-  __shared__ int s_win[kWavesPerBlock][kWin];
+  __shared__ int s_win[kWavesPerBlock][kWinAlloc];
   const int lane = threadIdx.x & 63;
   const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock +
                                                    (threadIdx.x >> 6));
@@ -396,7 +414,7 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
   constexpr int kLdsWaves = lds_waves<T, S>();
   extern __shared__ __align__(16) uint8_t s_dyn[];
   int *win_all = reinterpret_cast<int *>(s_dyn);
-  uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWin;
+  uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWinAlloc;
   // stage the packed x (x_bytes bytes, 16-byte aligned): whole 16-byte
   // units, then the tail byte by byte
   const uint4 *src = static_cast<const uint4 *>(a.B);
@@ -427,7 +445,7 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
   const int nw = gridDim.x * kLdsWaves;
   int c = (int)blockIdx.x * kLdsWaves + wave;
   if (c >= a.nchunks) return;  // uniform per wave; no barrier follows
-  int *win = win_all + wave * kWin;
+  int *win = win_all + wave * kWinAlloc;
   // Software pipeline over this wave's chunks c, c + nw, ...: while chunk c is
   // walked, the tiles and row window of c + nw and the partition entries of
   // c + 2nw are in flight.
@@ -594,7 +612,7 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
     const long long x_bytes = ceil_div(a.nbr, T) * T * (long long)sizeof(typename Tile<T>::W);
     const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;
     constexpr int kLdsWaves = lds_waves<T, S>();
-    const long long lds = (long long)sizeof(int) * kLdsWaves * kWin + xl;
+    const long long lds = (long long)sizeof(int) * kLdsWaves * kWinAlloc + xl;
     const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(typename Tile<T>::W));
     const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
     // automatic: T <= 16 only (T = 32 streams 132 B per tile and is

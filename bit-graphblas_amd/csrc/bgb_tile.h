@@ -303,6 +303,9 @@ __global__ __launch_bounds__(256) void chunk_rows_kernel(const int *__restrict__
 // after its chunk. When they do not fit (a chunk spanning more than kWin-2
 // block rows, e.g. long runs of empty rows) lookups go to memory.
 constexpr int kWin = 256;
+// LDS ints per wave: the window plus 64 for the group-row pass
+// (window_group_rows).
+constexpr int kWinAlloc = kWin + 64;
 struct RowWindow {
   const int *__restrict__ rowptr;
   int *win;
@@ -410,7 +413,7 @@ __device__ __forceinline__ int2 clamp_rows(int ia, int ib, bool last, int nbr) {
 // a mismatch (rowptr rewritten since the partition was made) the rows are
 // searched, the entry is rewritten (vector stores of the same value by every
 // lane) and the window restaged from memory. Must be called by all lanes.
-__device__ __forceinline__ void put_checked(RowWindow &rw, const int *__restrict__ rowptr, int *part, int *win,
+__device__ __forceinline__ bool put_checked(RowWindow &rw, const int *__restrict__ rowptr, int *part, int *win,
                                             int chunk, int nchunks, int nbr, long long k0, long long kend,
                                             int2 rows, int lane, const int (&wv)[kWin / 64]) {
   const bool last = chunk + 1 >= nchunks;
@@ -426,7 +429,9 @@ __device__ __forceinline__ void put_checked(RowWindow &rw, const int *__restrict
     const int Ia = ab.x, Ib = last ? nbr - 1 : ab.y;
     part[chunk] = Ia;
     rw.stage(rowptr, win, Ia, Ib, lane);
+    return true;
   }
+  return false;
 }
 
 // Value of lane (lane - D) inside the lane's 16-lane row (DPP row_shr:D, a
@@ -464,6 +469,48 @@ __device__ __forceinline__ uint64_t lane_up(uint64_t v, uint64_t fill) {
   const uint32_t lo = lane_up<D>((uint32_t)v, (uint32_t)fill);
   const uint32_t hi = lane_up<D>((uint32_t)(v >> 32), (uint32_t)(fill >> 32));
   return ((uint64_t)hi << 32) | lo;
+}
+
+// Row of the first tile of every lane group of a staged window, all groups at
+// once (group g of L lanes starts at tile k0 + g*S): with e_i = rowptr[Ia+i]
+// and h_i = the first group whose start tile is >= e_i, the row of group g is
+// Ia + max{i <= Ib - Ia : h_i <= g}. Each window entry (wv, as passed to
+// put) does one LDS max into a per-group slot, then a prefix max runs over
+// the lanes -- two LDS round trips instead of a binary search of ~log2(kWin)
+// dependent ones per lane. Must be called by all 64 lanes.
+template <int L, int S>
+__device__ __forceinline__ int window_group_rows(const RowWindow &rw, long long k0, int lane,
+                                                 const int (&wv)[kWin / 64]) {
+  static_assert((S & (S - 1)) == 0, "run length must be a power of two");
+  constexpr int G = 64 / L;
+  int *tmp = rw.win + kWin;
+  tmp[lane] = 0;  // h_0 = 0 (rowptr[Ia] <= k0), so every slot's answer is >= 0
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int last = rw.Ib - rw.Ia;  // entries 0 .. last start rows Ia .. Ib
+#pragma unroll
+  for (int j = 0; j < kWin / 64; ++j) {
+    const int i = lane + 64 * j;
+    const long long d = (long long)wv[j] - k0;
+    const long long h = d <= 0 ? 0 : (d + S - 1) / S;
+    if (i <= last && h < G) atomicMax(&tmp[(int)h], i);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int v = tmp[lane / L];
+  // inclusive prefix max over the lanes (groups are in lane order)
+  v = max(v, (int)dpp_shr<1>((uint32_t)v, 0u));
+  v = max(v, (int)dpp_shr<2>((uint32_t)v, 0u));
+  v = max(v, (int)dpp_shr<4>((uint32_t)v, 0u));
+  v = max(v, (int)dpp_shr<8>((uint32_t)v, 0u));
+  const int row = lane >> 4;
+  const int up1 = __shfl(v, (row * 16 - 1) & 63);  // last lane of row r-1
+  if (row >= 1) v = max(v, up1);
+  const int up2 = __shfl(v, (row * 16 - 17) & 63);  // last lane of row r-2 (rows r-3, r-2)
+  if (row >= 2) v = max(v, up2);
+  return rw.Ia + v;
 }
 
 // Inclusive segmented scan over lane groups of L lanes (L divides 16; keys

@@ -41,8 +41,9 @@ const Model kModels[4] = {
 // op 0 when the packed x is staged in LDS (persistent workgroups, T <= 16):
 // per-tile costs measured on R-MAT scale 20.
 const double kTileNsLds[4] = {0.0036, 0.0044, 0.0091, 0.0200};
-// LDS a workgroup may use on gfx950, minus the per-wave row windows (16 x 1 KB).
-constexpr long long kLdsForX = 160 * 1024 - 16 * 1024;
+// LDS a workgroup may use on gfx950, minus the per-wave row windows and
+// group-row slots (16 waves x 1280 B).
+constexpr long long kLdsForX = 160 * 1024 - 16 * 1280;
 
 // Packed-x bytes the staging kernel keeps in LDS for n rows at tile dim T
 // (T = 4: two nibbles per byte).

@@ -126,7 +126,7 @@ int bgb_get_run_length(int T);
 
 /* Where the bit-vector BMVs (bin_bin_bin, bin_bin_full, masked) read the
  * packed x for tile dim T: 0 = automatic (the whole packed x is staged in LDS
- * by one persistent workgroup per CU when it fits -- about 144 KB of packed
+ * by one persistent workgroup per CU when it fits -- about 140 KB of packed
  * x, i.e. n up to ~1.1M -- and the tile stream is large next to it), 1 =
  * always gathered from memory, 2 = staged whenever it fits. Process-wide
  * tuning knob (no reference counterpart). Default 0. */
