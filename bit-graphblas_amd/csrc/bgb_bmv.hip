@@ -269,9 +269,44 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   int e = active ? row_end(I) : INT_MAX;
   int e2 = active ? row_end(I + 1) : INT_MAX;
 
+  // T = 4 bit ops, branch-free: a tile's four row counts are byte-wise
+  // popcounts of its word (row r in byte r), accumulated packed (a run adds
+  // at most 8 x 4 per byte); a run with at most one row end keeps a snapshot
+  // of the packed counts at that end instead of branching per tile. Waves in
+  // which some lane's run holds a second row end (or an empty row) take the
+  // general walk below.
+  bool walked = false;
+  if constexpr (T == 4 && !is_full(OP) && ST) {
+    const long long lastpos = pa + n;
+    const bool multi = active && (long long)e2 <= lastpos;
+    if (__ballot(multi) == 0) {
+      uint32_t pk = 0, snap = 0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        // X[s] is 0 past the run, so those tiles add nothing
+        uint32_t m = F[s][0] & ((uint32_t)X[s] * 0x01010101u);
+        m = m - ((m >> 1) & 0x55555555u);
+        m = (m & 0x33333333u) + ((m >> 2) & 0x33333333u);
+        pk += m;
+        snap = (int)(pa + s) + 1 == e ? pk : snap;
+      }
+      const bool ends = active && (long long)e <= lastpos;
+      const uint32_t open = ends ? pk - snap : pk;  // bytes never borrow: snap <= pk per byte
+#pragma unroll
+      for (int i = 0; i < R; ++i) acc[i] = (Acc)((open >> (8 * i)) & 0xFFu);
+      if (ends) {
+        held = I;
+#pragma unroll
+        for (int i = 0; i < R; ++i) hv[i] = (Acc)((snap >> (8 * i)) & 0xFFu);
+        ++I;
+      }
+      walked = true;
+    }
+  }
+
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (s < n) {
+    if (!walked && s < n) {
       if constexpr (kSeg) {
         // ascending columns, adding 0 for clear bits: the CSR summation order
 #pragma unroll
