@@ -208,10 +208,10 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
   for (int i = 0; i < R; ++i) sc[i] = acc[i];
   group_scan<L, R, uint64_t>(key, sc, [](uint64_t x, uint64_t y) { return x | y; });
-  const int kin = __shfl_up(key, L);
+  const int kin = group_up<L>(key);
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    const uint64_t up = __shfl_up(sc[i], L);
+    const uint64_t up = group_up<L, uint64_t>(sc[i]);
     if (g > 0 && kin == I0) hv[i] |= up;
   }
   if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any);

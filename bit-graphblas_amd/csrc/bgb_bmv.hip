@@ -373,11 +373,11 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   for (int i = 0; i < R; ++i) sc[i] = acc[i];
   group_scan<L, R, Acc>(key, sc, [](Acc x, Acc y) { return x + y; });
   // Carry into this group's first flushed row from the groups before it.
-  const int kin = __shfl_up(key, L);
+  const int kin = group_up<L>(key);
   Acc cin[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    const Acc up = __shfl_up(sc[i], L);
+    const Acc up = group_up<L, Acc>(sc[i]);
     cin[i] = (g > 0 && kin == I0) ? up : (Acc)0;
   }
   if (held >= 0) {

@@ -338,13 +338,13 @@ struct RowWindow {
     Ib = ib;
     const int m = ib + 2 - ia;
     staged = m <= kWin;
-    if (staged) {
+    // Every slot is written, staged or not (entries past the window are not
+    // read): a load whose value is consumed only on some paths stays pending
+    // at the join, and the first reuse of its register then waits for every
+    // load in flight.
 #pragma unroll
-      for (int j = 0; j < kWin / 64; ++j) {
-        const int i = lane + 64 * j;
-        if (i < m) win[i] = wv[j];
-      }
-    }
+    for (int j = 0; j < kWin / 64; ++j) win[lane + 64 * j] = wv[j];
+    win[kWin + lane] = 0;  // group-row slots (window_group_rows), cleared under the same wait
     // the wave reads entries other lanes wrote (wave-private LDS, in order)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -421,8 +421,23 @@ __device__ __forceinline__ bool put_checked(RowWindow &rw, const int *__restrict
   bool ok = false;
   rw.dispatch([&](auto st) {
     constexpr bool ST = decltype(st)::value;
-    ok = rw.at_t<ST>(rows.x) <= k0 && k0 < rw.at_t<ST>(rows.x + 1);
-    if (!last) ok = ok && rw.at_t<ST>(rows.y) <= kend && kend < rw.at_t<ST>(rows.y + 1);
+    if constexpr (ST) {
+      // the four entries are in the lanes' registers (entry i in lane i % 64,
+      // slot i / 64): each lane checks the ones it holds, one ballot joins them
+      const int mlast = rows.y + 1 - rows.x;  // entry of rowptr[Ib + 1]
+      bool bad = false;
+#pragma unroll
+      for (int j = 0; j < kWin / 64; ++j) {
+        const int i = lane + 64 * j;
+        const long long e = wv[j];
+        bad |= (i == 0 && e > k0) || (i == 1 && e <= k0);
+        if (!last) bad |= (i == mlast - 1 && e > kend) || (i == mlast && e <= kend);
+      }
+      ok = __ballot(bad) == 0;
+    } else {
+      ok = rw.at_t<ST>(rows.x) <= k0 && k0 < rw.at_t<ST>(rows.x + 1);
+      if (!last) ok = ok && rw.at_t<ST>(rows.y) <= kend && kend < rw.at_t<ST>(rows.y + 1);
+    }
   });
   if (!__builtin_amdgcn_readfirstlane((int)ok)) {
     const int2 ab = wave_find_rows2(rowptr, nbr - 1, k0, last ? k0 : kend, lane);
@@ -471,6 +486,42 @@ __device__ __forceinline__ uint64_t lane_up(uint64_t v, uint64_t fill) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Row broadcasts (GFX9 DPP row_bcast): Bcast15 gives rows 1 and 3 the value of
+// lanes 15 and 47, Bcast31 gives rows 2 and 3 the value of lane 31; other
+// lanes get `old`. Together with in-row row_shr steps they finish a
+// 64-lane scan without LDS traffic. WaveShr1: the value of lane - 1 (lane 0
+// gets `old`).
+enum DppWide { kBcast15 = 0x142, kBcast31 = 0x143, kWaveShr1 = 0x138 };
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_wide(uint32_t v, uint32_t old) {
+  constexpr int kRowMask = CTRL == kBcast15 ? 0xA : CTRL == kBcast31 ? 0xC : 0xF;
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, kRowMask, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_wide(float v, float old) {
+  return __uint_as_float(dpp_wide<CTRL>(__float_as_uint(v), __float_as_uint(old)));
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_wide(uint64_t v, uint64_t old) {
+  const uint32_t lo = dpp_wide<CTRL>((uint32_t)v, (uint32_t)old);
+  const uint32_t hi = dpp_wide<CTRL>((uint32_t)(v >> 32), (uint32_t)(old >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Value of lane (lane - L) for lanes >= L (others get `old`): the previous
+// lane group's lane in the same position. DPP wave_shr:1 for L = 1,
+// ds_bpermute otherwise.
+template <int L>
+__device__ __forceinline__ int group_up(int v) {
+  if constexpr (L == 1) return (int)dpp_wide<kWaveShr1>((uint32_t)v, 0u);
+  else return __shfl_up(v, L);
+}
+template <int L, class V>
+__device__ __forceinline__ V group_up(V v) {
+  if constexpr (L == 1) return dpp_wide<kWaveShr1>(v, (V)0);
+  else return __shfl_up(v, L);
+}
+
 // Row of the first tile of every lane group of a staged window, all groups at
 // once (group g of L lanes starts at tile k0 + g*S): with e_i = rowptr[Ia+i]
 // and h_i = the first group whose start tile is >= e_i, the row of group g is
@@ -483,11 +534,9 @@ __device__ __forceinline__ int window_group_rows(const RowWindow &rw, long long 
                                                  const int (&wv)[kWin / 64]) {
   static_assert((S & (S - 1)) == 0, "run length must be a power of two");
   constexpr int G = 64 / L;
+  // put() cleared the slots; h_0 = 0 (rowptr[Ia] <= k0), so every group's
+  // answer is >= 0
   int *tmp = rw.win + kWin;
-  tmp[lane] = 0;  // h_0 = 0 (rowptr[Ia] <= k0), so every slot's answer is >= 0
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int last = rw.Ib - rw.Ia;  // entries 0 .. last start rows Ia .. Ib
 #pragma unroll
   for (int j = 0; j < kWin / 64; ++j) {
@@ -505,11 +554,8 @@ __device__ __forceinline__ int window_group_rows(const RowWindow &rw, long long 
   v = max(v, (int)dpp_shr<2>((uint32_t)v, 0u));
   v = max(v, (int)dpp_shr<4>((uint32_t)v, 0u));
   v = max(v, (int)dpp_shr<8>((uint32_t)v, 0u));
-  const int row = lane >> 4;
-  const int up1 = __shfl(v, (row * 16 - 1) & 63);  // last lane of row r-1
-  if (row >= 1) v = max(v, up1);
-  const int up2 = __shfl(v, (row * 16 - 17) & 63);  // last lane of row r-2 (rows r-3, r-2)
-  if (row >= 2) v = max(v, up2);
+  v = max(v, (int)dpp_wide<kBcast15>((uint32_t)v, 0u));
+  v = max(v, (int)dpp_wide<kBcast31>((uint32_t)v, 0u));
   return rw.Ia + v;
 }
 
@@ -538,6 +584,27 @@ __device__ __forceinline__ void group_scan(int key, Acc (&sc)[R], Op op) {
   BGB_ROW_STEP(4 * L)
   BGB_ROW_STEP(8 * L)
 #undef BGB_ROW_STEP
+  if constexpr (L == 1) {
+    // one lane per group: the last group of rows r-1 / r-2 is lane 15, 31 or
+    // 47, which the row broadcasts deliver (rows 1, 3 then rows 2, 3)
+    {
+      const int ku = (int)dpp_wide<kBcast15>((uint32_t)key, kNone);
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const Acc up = dpp_wide<kBcast15>(sc[i], (Acc)0);
+        if (ku == key) sc[i] = op(sc[i], up);
+      }
+    }
+    {
+      const int ku = (int)dpp_wide<kBcast31>((uint32_t)key, kNone);
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const Acc up = dpp_wide<kBcast31>(sc[i], (Acc)0);
+        if (ku == key) sc[i] = op(sc[i], up);
+      }
+    }
+    return;
+  }
   const int lane = threadIdx.x & 63, row = lane >> 4;
   // last group of row r-1, same lane position inside the group
   const int src1 = row * 16 - L + (lane & (L - 1));
