@@ -178,3 +178,26 @@ def test_tile_stats_match_oracle(orc, dev, T):
         # sorted rows: the estimate counts the non-zero row words exactly
         assert est == int(wnzr.sum())
         assert est >= A.nblocks
+
+
+@pytest.mark.parametrize("T", (4, 8, 16, 32))
+def test_directed_bfs_pulls_over_the_transpose(orc, dev, T):
+    """Directed graphs: the batched BFS pulls over in-edges, i.e. the rows of A^T,
+    so BFS over the transposed B2SR gives the forward (out-edge) levels of A --
+    the oracle's queue BFS over A's CSR. The BFS over A itself gives the levels
+    of the reversed graph (the oracle's BFS over A^T)."""
+    from bgb import b2sr as B
+    rng = np.random.default_rng(100 + T)
+    n = 2500
+    csr = directed_csr(orc, rng, n, n, 4 * n)
+    log(f"[case] directed bfs T={T} nnz={csr.nnz}")
+    A = B.B2SR.from_csr(csr, T, dev)
+    At = A.transpose()
+    src = rng.integers(0, n, 64)
+    lev, _ = B.bfs_multi(At, src)
+    want = np.stack([orc.bfs(csr.rowptr, csr.colind, int(s))[0] for s in src])
+    np.testing.assert_array_equal(lev.cpu().numpy(), want)
+    cp, ri, _ = orc.csr2csc(n, n, csr.rowptr, csr.colind)
+    lev_rev, _ = B.bfs_multi(A, src)
+    want_rev = np.stack([orc.bfs(cp, ri, int(s))[0] for s in src])
+    np.testing.assert_array_equal(lev_rev.cpu().numpy(), want_rev)
