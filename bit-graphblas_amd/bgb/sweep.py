@@ -13,7 +13,7 @@ rocSPARSE csrmv here).
 This module does the same over .mtx files already on disk (nothing is
 downloaded):
 
-    python -m bgb.sweep bin_bin_full a.mtx b.mtx --tiles 4,8,16,32 [--transpose 1]
+    PYTHONPATH=bit-graphblas_amd python -m bgb.sweep bin_bin_full a.mtx b.mtx --tiles 4,8,16,32 [--transpose 1]
 
 Inputs follow the drivers: readMtx with directed = 0 (bgb_read_mtx), an
 optional transpose (argv[2] of the drivers; csr2csc on the GPU), a random 0/1
