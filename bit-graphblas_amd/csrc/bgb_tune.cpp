@@ -31,16 +31,16 @@ struct Model {
 // bin_full_full (FP32), op 2: K=16 MFMA, op 3: one level of the batched
 // 64-source BFS. Fitted to MI355X runs on R-MAT scale 20 / 22 (bit ops) and the
 // 4096^2 grid (FP32): T <= 16 is bound by per-tile issue and latency, T = 32 by
-// HBM at ~5.3 TB/s.
+// HBM at ~5.5 TB/s.
 const Model kModels[4] = {
-    {{0.0046, 0.0057, 0.0102, 0.0200}, {0.0010, 0.0010, 0.0010, 0.0010}, 5300.0},
+    {{0.0045, 0.0048, 0.0091, 0.0200}, {0.0010, 0.0010, 0.0010, 0.0010}, 5500.0},
     {{0.0055, 0.0095, 0.0450, 0.0580}, {0.0010, 0.0010, 0.0010, 0.0010}, 4000.0},
     {{1e9, 1e9, 0.0600, 0.2000}, {1e9, 1e9, 0.0300, 0.0600}, 4000.0},
     {{0.0117, 0.0160, 0.0300, 0.0700}, {0.0050, 0.0080, 0.0150, 0.0300}, 4000.0},
 };
 // op 0 when the packed x is staged in LDS (persistent workgroups, T <= 16):
 // per-tile costs measured on R-MAT scale 20.
-const double kTileNsLds[4] = {0.0036, 0.0044, 0.0091, 0.0200};
+const double kTileNsLds[4] = {0.0035, 0.0044, 0.0092, 0.0200};
 // LDS a workgroup may use on gfx950, minus the per-wave row windows and
 // group-row slots (16 waves x 1280 B).
 constexpr long long kLdsForX = 160 * 1024 - 16 * 1280;
