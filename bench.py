@@ -199,21 +199,31 @@ def bfs_section(torch, B, graph, csr, T, nsrc=64, reps=3, plain=True):
     deg = csr.degrees()
     rng = np.random.default_rng(64)
     src = rng.choice(np.nonzero(deg)[0], nsrc, replace=False).astype(np.int32)
-    lev, nl = B.bfs_multi(A, src)  # warm-up (also the result)
-    torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        lev, nl = B.bfs_multi(A, src)
-    e.record()
-    torch.cuda.synchronize()
-    t_batch = s.elapsed_time(e) / 1e3 / reps
+
+    def timed(push):
+        lev, nl = B.bfs_multi(A, src, push=push)  # warm-up (also the result)
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(reps):
+            lev, nl = B.bfs_multi(A, src, push=push)
+        e.record()
+        torch.cuda.synchronize()
+        return lev, nl, s.elapsed_time(e) / 1e3 / reps
+
+    # the graph is symmetric: A is its own transpose, so it is also the push matrix
+    lev_pull, _, t_pull = timed(None)
+    lev, nl, t_batch = timed(A)
     # Graph500 numerator: undirected edges in each source's component
     dg = torch.from_numpy(deg.astype(np.int64)).to(dev)
     edges = int(((lev >= 0).to(torch.int64) * dg).sum().item()) // 2
     out = {"tile_dim": T, "sources": nsrc, "levels": nl, "edges_traversed": edges,
-           "batched_ms": round(t_batch * 1e3, 3), "batched_gteps": round(edges / t_batch / 1e9, 3)}
-    log(f"[bench] BFS batched T={T}: {t_batch * 1e3:.3f} ms, {nl} levels, {edges / t_batch / 1e9:.2f} GTEPS")
+           "batched_ms": round(t_batch * 1e3, 3), "batched_gteps": round(edges / t_batch / 1e9, 3),
+           "direction": "push levels for small frontiers, pull otherwise",
+           "batched_pull_only_ms": round(t_pull * 1e3, 3),
+           "pull_only_identical": bool(torch.equal(lev_pull, lev))}
+    log(f"[bench] BFS batched T={T}: {t_batch * 1e3:.3f} ms, {nl} levels, {edges / t_batch / 1e9:.2f} GTEPS "
+        f"(pull only {t_pull * 1e3:.3f} ms)")
     if plain:
         plv = torch.empty((nsrc, csr.nrows), dtype=torch.int32, device=dev)
         B.bfs_plain(A, int(src[0]), out=plv[0])  # warm-up

@@ -60,6 +60,8 @@ SIGNATURES = {
     "bgb_k16_batched": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "bgb_bfs_multi": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int,
                                _vp, _vp]),
+    "bgb_bfs_multi_dir": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_int,
+                                   _vp, _c_int, _vp, _vp]),
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_autotune_tiledim": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp]),

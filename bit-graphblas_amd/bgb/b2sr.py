@@ -196,14 +196,23 @@ def bmv_bin_full_full_bf16(A, x_bf16, out=None, accumulate=False):
     return C
 
 
-def bfs_multi(A, sources, use_graph=True):
-    """Batched BFS from up to 64 sources; returns (levels[nsrc, n] int32 tensor, nlevels)."""
+def bfs_multi(A, sources, use_graph=True, push=None):
+    """Batched BFS from up to 64 sources; returns (levels[nsrc, n] int32 tensor, nlevels).
+
+    push: the B2SR of A^T (A itself when A is symmetric) to switch small-frontier
+    levels to push (bgb_bfs_multi_dir); None pulls every level."""
     src = np.ascontiguousarray(sources, dtype=np.int32)
     n = A.nrows
     lev = torch.empty((src.size, n), dtype=torch.int32, device=A.device)
     nl = ctypes.c_int(0)
-    call("bgb_bfs_multi", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
-         src.ctypes.data, src.size, _ptr(lev), int(use_graph), ctypes.byref(nl), _stream())
+    if push is None:
+        call("bgb_bfs_multi", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks, n,
+             src.ctypes.data, src.size, _ptr(lev), int(use_graph), ctypes.byref(nl), _stream())
+    else:
+        assert push.T == A.T and push.nblockrows == A.nblockrows
+        call("bgb_bfs_multi_dir", A.T, _ptr(A.val), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows, A.nblocks,
+             _ptr(push.val), _ptr(push.rowptr), _ptr(push.colind), n, src.ctypes.data, src.size, _ptr(lev),
+             int(use_graph), ctypes.byref(nl), _stream())
     return lev, nl.value
 
 

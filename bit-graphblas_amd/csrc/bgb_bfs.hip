@@ -20,10 +20,22 @@
 //   per graph, so the double buffers alternate) is captured in a hipGraph
 //   and replayed until a level finds nothing; device-side flags turn the
 //   levels after the last one into no-ops.
+//   Direction: a pull pass streams every tile, whatever the frontier size.
+//   Given the B2SR of A^T (A's own arrays when A is symmetric), a level whose
+//   frontier is cheap to push is a push instead: one wave per block row of
+//   A^T holding frontier vertices walks its tiles once per frontier row --
+//   the vertices v with A[v][u] = 1, which pull from u -- and ORs F[u] into
+//   the unvisited ones with atomics. The push cost of a frontier is the tiles
+//   of the A^T block rows of its vertices (one walk per vertex); each level
+//   sums it for the vertices it adds, and the end-of-level kernel pushes the
+//   next level when that sum is below nblocks / kPushDiv (a pull streams all
+//   nblocks tiles; a pushed tile costs far more: a random atomic per
+//   neighbour). The first level (the sources) is a push.
 // The tile walk, group scan and chunk carries are those of the BMV kernels,
 // with OR as the combine operator.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -36,6 +48,10 @@ namespace {
 constexpr int kBfsWaves = 4;
 constexpr int kBfsRun = 8;
 constexpr int kPlanes = 16;  // level < 2^16
+constexpr int kPushDiv = 64;    // push while the next frontier's push cost < nblocks / kPushDiv tiles
+constexpr int kCounters = 256;  // push-cost counters, spread to avoid one hot address
+constexpr int kPushSeg = 2048;  // push: frontier rows of longer block rows are split into segments
+constexpr int kBigCap = 1 << 16;  // push: (block row, row) entries of the long-row list
 
 __device__ __forceinline__ long long lmin2(long long a, long long b) { return a < b ? a : b; }
 
@@ -52,10 +68,16 @@ struct BfsState {
   uint64_t *P;           // kPlanes bit-planes, stride npad
   long long npad;        // NBR * T
   uint64_t full;         // all source bits
-  int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done
+  int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done, [3] push level
+  unsigned long long *fcnt;  // kCounters counters of the next frontier's push cost (tiles)
   int *carry_key;
   uint64_t *carry_val;
   const int *part;       // part[c]: block row of chunk c's first tile
+  const void *At;        // B2SR of A^T for push levels (null: pull only)
+  const int *rowptrT;
+  const int *colindT;
+  int *big_n;            // push: entries in big_list
+  int2 *big_list;        // push: frontier rows (I, r) of block rows longer than kPushSeg tiles
 };
 
 // Records the vertices first reached at this level (new bits) for one row.
@@ -74,7 +96,8 @@ __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint6
 
 template <int T>
 __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const uint64_t (&acc)[Tile<T>::R],
-                                          const uint64_t (&vr)[Tile<T>::R], int depth, bool &any) {
+                                          const uint64_t (&vr)[Tile<T>::R], int depth, bool &any,
+                                          unsigned long long &cost) {
   constexpr int R = Tile<T>::R, L = Tile<T>::L;
   using W = typename Tile<T>::W;
   uint32_t bits = 0;
@@ -87,6 +110,8 @@ __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const
       bits |= 1u << (T - 1 - (q * R + i));
     }
   }
+  // push cost of the rows added: one walk of A^T's block row I each
+  if (b.rowptrT && bits) cost += (unsigned long long)__popc(bits) * (b.rowptrT[I + 1] - b.rowptrT[I]);
 #pragma unroll
   for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
   if (q == 0) static_cast<W *>(b.fnzn)[I] = (W)bits;
@@ -104,7 +129,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
   __shared__ int s_win[kBfsWaves][kWinAlloc];
-  if (b.ctl[2]) return;  // BFS already finished: this captured level is a no-op
+  if (b.ctl[2] || b.ctl[3]) return;  // BFS finished, or a push level: no-op
   const int depth = b.ctl[0];
   const int lane = threadIdx.x & 63;
   const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves +
@@ -143,13 +168,14 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   const int I0 = I;
 
   bool any = false;
+  unsigned long long nnew = 0;  // push cost of the rows this wave adds
   uint64_t acc[R], zero[R], vr[R], hv[R], hvr[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) acc[i] = zero[i] = hv[i] = hvr[i] = 0;
   if (active && pa == 0)
     for (int r = 0; r < I0; ++r) {
       load_visited<T>(b, r, q, vr);
-      bfs_flush<T>(b, r, q, zero, vr, depth, any);
+      bfs_flush<T>(b, r, q, zero, vr, depth, any, nnew);
     }
   if (active) load_visited<T>(b, I, q, vr);
   int held = -1;
@@ -185,7 +211,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
             hvr[i] = vr[i];
           }
         } else {
-          bfs_flush<T>(b, I, q, acc, vr, depth, any);
+          bfs_flush<T>(b, I, q, acc, vr, depth, any, nnew);
         }
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
@@ -194,7 +220,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
         e2 = row_end(I + 1);
         while (e == pos) {
           load_visited<T>(b, I, q, vr);
-          bfs_flush<T>(b, I, q, zero, vr, depth, any);
+          bfs_flush<T>(b, I, q, zero, vr, depth, any, nnew);
           ++I;
           e = e2;
           e2 = row_end(I + 1);
@@ -214,7 +240,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     const uint64_t up = group_up<L, uint64_t>(sc[i]);
     if (g > 0 && kin == I0) hv[i] |= up;
   }
-  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any);
+  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any, nnew);
   const int last = (int)lmin2(G - 1, (b.nblocks - 1 - k0) / S);
   if (g == last) {
     const bool open = key < b.nbr;
@@ -225,6 +251,9 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   }
   if (__ballot(any) != 0 && lane == 0) b.ctl[1] = 1;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) nnew += __shfl_xor(nnew, m);
+  if (lane == 0 && nnew) atomicAdd(b.fcnt + (chunk & (kCounters - 1)), nnew);
   });
 }
 
@@ -232,7 +261,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 template <int T>
 __global__ void bfs_fixup_kernel(BfsState b) {
   using W = typename Tile<T>::W;
-  if (b.ctl[2]) return;
+  if (b.ctl[2] || b.ctl[3]) return;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= (long long)b.nchunks * T) return;
   const long long c = tid / T;
@@ -246,7 +275,9 @@ __global__ void bfs_fixup_kernel(BfsState b) {
   const uint64_t nb = cv & ~vold;
   if (!nb) return;
   const int depth = b.ctl[0];
-  atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
+  const unsigned long long fold = atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
+  if (!fold && b.rowptrT)
+    atomicAdd(b.fcnt + (tid & (kCounters - 1)), (unsigned long long)(b.rowptrT[key + 1] - b.rowptrT[key]));
   atomicOr(reinterpret_cast<unsigned long long *>(b.V + v), (unsigned long long)nb);
   for (int k = 0; k < kPlanes; ++k)
     if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), (unsigned long long)nb);
@@ -256,12 +287,152 @@ __global__ void bfs_fixup_kernel(BfsState b) {
   b.ctl[1] = 1;
 }
 
-// End of a level: stop when nothing new was found, else advance the depth.
-__global__ void bfs_advance_kernel(int *ctl) {
-  if (ctl[2]) return;
+// End of a level: stop when nothing new was found, else advance the depth and
+// pick the next level's direction from the push cost of the vertices this
+// level added (push below push_limit). One block of kCounters threads.
+__global__ void bfs_advance_kernel(int *ctl, unsigned long long *fcnt, unsigned long long push_limit) {
+  __shared__ unsigned long long part[kCounters];
+  const int t = threadIdx.x;
+  part[t] = fcnt[t];
+  fcnt[t] = 0;
+  __syncthreads();
+  for (int m = kCounters / 2; m > 0; m >>= 1) {
+    if (t < m) part[t] += part[t + m];
+    __syncthreads();
+  }
+  if (t != 0 || ctl[2]) return;
   if (ctl[1] == 0) ctl[2] = 1;
   else ctl[0] += 1;
   ctl[1] = 0;
+  ctl[3] = part[0] < push_limit ? 1 : 0;
+}
+
+// Push levels write the next frontier and its packed non-empty bits with
+// atomics: both start cleared (a pull level overwrites every entry instead).
+__global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
+  if (b.ctl[2] || !b.ctl[3]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *b.big_n = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < b.npad; i += stride) b.Fn[i] = 0;
+  uint8_t *z = static_cast<uint8_t *>(b.fnzn);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < zbytes; i += stride) z[i] = 0;
+}
+
+// Pushes frontier row r of A^T's block row I over its tiles [k0, k1) (lanes
+// stride over them): F[u] & ~V[v] is OR-ed into Fn[v] for every neighbour v
+// -- one atomic per pushed edge. V still holds the previous levels only, so
+// Fn collects exactly this level's new bits; bfs_push_finish_kernel then
+// folds them into V, the level planes and the packed non-empty words.
+template <int T>
+__device__ __forceinline__ void push_row_segment(const BfsState &b, long long I, int r, int k0, int k1,
+                                                 int lane) {
+  using W = typename Tile<T>::W;
+  const uint64_t Fu = b.F[I * T + r];
+  for (int k = k0 + lane; k < k1; k += 64) {
+    uint32_t w = (uint32_t) static_cast<const W *>(b.At)[(long long)k * T + r];
+    const long long J = b.colindT[k];
+    while (w) {
+      const int hh = 31 - __clz(w);
+      w &= ~(1u << hh);
+      const long long v = J * T + (T - 1 - hh);
+      const uint64_t nb = Fu & ~b.V[v];
+      if (nb) atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
+    }
+  }
+}
+
+// End of a push pass: the any-new flag and the wave's push-cost sum.
+__device__ __forceinline__ void push_finish(const BfsState &b, bool any, unsigned long long cost, int lane,
+                                            int slot) {
+  if (__ballot(any) != 0 && lane == 0) b.ctl[1] = 1;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) cost += __shfl_xor(cost, m);
+  if (lane == 0 && cost) atomicAdd(b.fcnt + (slot & (kCounters - 1)), cost);
+}
+
+// Push level, part 1: one wave per block row of A^T (grid-stride) whose
+// packed frontier word is non-zero. Rows of block rows up to kPushSeg tiles
+// are pushed here; those of longer block rows go to big_list (pushed whole
+// here only if the list is full).
+template <int T>
+__global__ __launch_bounds__(256) void bfs_push_kernel(BfsState b) {
+  using W = typename Tile<T>::W;
+  if (b.ctl[2] || !b.ctl[3]) return;
+  const int lane = threadIdx.x & 63;
+  const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long I = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); I < b.nbr; I += waves) {
+    uint32_t fw = (uint32_t) static_cast<const W *>(b.fnz)[I];
+    if (!fw) continue;
+    const int ks = b.rowptrT[I], ke = b.rowptrT[I + 1];
+    while (fw) {
+      const int h = 31 - __clz(fw);
+      fw &= ~(1u << h);
+      const int r = T - 1 - h;
+      if (ke - ks > kPushSeg) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(b.big_n, 1);
+        slot = __shfl(slot, 0);
+        if (slot < kBigCap) {
+          if (lane == 0) b.big_list[slot] = make_int2((int)I, r);
+          continue;
+        }
+      }
+      push_row_segment<T>(b, I, r, ks, ke, lane);
+    }
+  }
+}
+
+// Push level, part 2: the long rows, in kPushSeg-tile segments spread over
+// all waves (segment s of an entry goes to wave s mod waves).
+template <int T>
+__global__ __launch_bounds__(256) void bfs_push_long_kernel(BfsState b) {
+  if (b.ctl[2] || !b.ctl[3]) return;
+  const int lane = threadIdx.x & 63;
+  const int nbig = min(*b.big_n, kBigCap);
+  const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  for (int i = 0; i < nbig; ++i) {
+    const int2 e = b.big_list[i];
+    const int ks = b.rowptrT[e.x], ke = b.rowptrT[e.x + 1];
+    const long long nseg = (ke - ks + kPushSeg - 1) / kPushSeg;
+    for (long long sg = wave; sg < nseg; sg += waves) {
+      const int k0 = ks + (int)sg * kPushSeg;
+      push_row_segment<T>(b, e.x, e.y, k0, min(ke, k0 + kPushSeg), lane);
+    }
+  }
+}
+
+// Push level, part 3: one thread per block row folds the new bits Fn into V
+// and the level planes, writes the packed non-empty word, and sums the push
+// cost of the vertices added (for the next level's direction).
+template <int T>
+__global__ void bfs_push_finish_kernel(BfsState b) {
+  using W = typename Tile<T>::W;
+  if (b.ctl[2] || !b.ctl[3]) return;
+  const int depth = b.ctl[0];
+  const int lane = threadIdx.x & 63;
+  bool any = false;
+  unsigned long long cost = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x; I < b.nbr; I += stride) {
+    uint32_t bits = 0;
+#pragma unroll 4
+    for (int r = 0; r < T; ++r) {
+      const long long v = I * T + r;
+      const uint64_t nb = b.Fn[v];
+      if (!nb) continue;
+      b.V[v] |= nb;
+      for (int k = 0; k < kPlanes; ++k)
+        if ((depth >> k) & 1) b.P[k * b.npad + v] |= nb;
+      bits |= 1u << (T - 1 - r);
+    }
+    static_cast<W *>(b.fnzn)[I] = (W)bits;
+    if (bits) {
+      any = true;
+      cost += (unsigned long long)__popc(bits) * (b.rowptrT[I + 1] - b.rowptrT[I]);
+    }
+  }
+  push_finish(b, any, cost, lane, blockIdx.x);
 }
 
 template <class V>
@@ -335,14 +506,15 @@ unsigned nblk(long long n, int b = 256) { return (unsigned)ceil_div(n > 0 ? n : 
 unsigned nfill(long long n) { return capped_blocks(n, 256); }
 
 template <int T>
-int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
-                   const int *h_src, int nsrc, int *level, int use_graph, int *h_nlevels, hipStream_t s) {
+int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, const void *At,
+                   const int *rowptrT, const int *colindT, int n, const int *h_src, int nsrc, int *level,
+                   int use_graph, int *h_nlevels, hipStream_t s) {
   constexpr int S = kBfsRun, CH = (64 / Tile<T>::L) * S;
   const long long npad = (long long)nbr * T;
   const long long nwords = ceil_div(nbr, T) * T;
   const int wb = word_bytes(T);
   const long long nchunks = ceil_div(nblocks, CH);
-  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv, pt;
+  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv, pt, fc, bign, bigl;
   BGB_HIP_TRY(F0.alloc(8 * npad, s));
   BGB_HIP_TRY(F1.alloc(8 * npad, s));
   BGB_HIP_TRY(Vb.alloc(8 * npad, s));
@@ -354,6 +526,9 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   BGB_HIP_TRY(ck.alloc(sizeof(int) * (nchunks + 1), s));
   BGB_HIP_TRY(cv.alloc(8 * (nchunks + 1) * T, s));
   BGB_HIP_TRY(pt.alloc(sizeof(int) * (nchunks + 1), s));
+  BGB_HIP_TRY(fc.alloc(sizeof(unsigned long long) * kCounters, s));
+  BGB_HIP_TRY(bign.alloc(sizeof(int), s));
+  BGB_HIP_TRY(bigl.alloc(sizeof(int2) * (At ? kBigCap : 1), s));
   if (nchunks > 0)  // chunk -> first block row, once for every level
     chunk_rows_kernel<<<nblk(nbr), 256, 0, s>>>(rowptr, nbr, CH, (int)nchunks, pt.as<int>());
   const uint64_t full = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
@@ -366,7 +541,9 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     bfs_fill_kernel<uint64_t><<<nfill(npad - n), 256, 0, s>>>(Vb.as<uint64_t>() + n, npad - n, full);
   bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z0.as<uint8_t>(), wb * nwords + 16, 0);
   bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z1.as<uint8_t>(), wb * nwords + 16, 0);
-  const int init_ctl[4] = {1, 0, 0, 0};
+  bfs_fill_kernel<unsigned long long><<<1, kCounters, 0, s>>>(fc.as<unsigned long long>(), kCounters, 0ull);
+  const bool push = At != nullptr;
+  const int init_ctl[4] = {1, 0, 0, push ? 1 : 0};  // depth 1: a push level (the frontier is the sources)
   BGB_HIP_TRY(hipMemcpyAsync(ctl.ptr, init_ctl, sizeof(init_ctl), hipMemcpyHostToDevice, s));
   BGB_HIP_TRY(hipMemcpyAsync(src.ptr, h_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, s));
   bfs_seed_kernel<T><<<1, 64, 0, s>>>(src.as<int>(), nsrc, F0.as<uint64_t>(), Vb.as<uint64_t>(), z0.ptr);
@@ -393,16 +570,31 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     b.carry_key = ck.as<int>();
     b.carry_val = cv.as<uint64_t>();
     b.part = pt.as<int>();
+    b.fcnt = fc.as<unsigned long long>();
+    b.At = At;
+    b.rowptrT = rowptrT;
+    b.colindT = colindT;
+    b.big_n = bign.as<int>();
+    b.big_list = bigl.as<int2>();
   }
   const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
   const unsigned fblocks = nblk(nchunks * T);
+  const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 1 << 16);
+  const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
+  const long long zbytes = wb * nwords + 16;
   auto level_pair = [&]() {
     for (int p = 0; p < 2; ++p) {
+      bfs_clear_kernel<<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
       if (nchunks > 0) {
         bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
         bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
+        if (push) {
+          bfs_push_kernel<T><<<pblocks, 256, 0, s>>>(st[p]);
+          bfs_push_long_kernel<T><<<1024, 256, 0, s>>>(st[p]);
+          bfs_push_finish_kernel<T><<<nblk(nbr), 256, 0, s>>>(st[p]);
+        }
       }
-      bfs_advance_kernel<<<1, 1, 0, s>>>(ctl.as<int>());
+      bfs_advance_kernel<<<1, kCounters, 0, s>>>(ctl.as<int>(), fc.as<unsigned long long>(), push_limit);
     }
   };
   int h_ctl[4] = {0, 0, 0, 0};
@@ -534,18 +726,28 @@ extern "C" {
 int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
                   int nblocks, int n, const int *h_sources, int nsrc, int *d_levels, int use_graph,
                   int *h_nlevels, bgb_stream_t stream) {
+  return bgb_bfs_multi_dir(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, nullptr, nullptr, nullptr, n,
+                           h_sources, nsrc, d_levels, use_graph, h_nlevels, stream);
+}
+
+int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
+                      int nblocks, const void *d_At, const int *d_rowptrT, const int *d_colindT, int n,
+                      const int *h_sources, int nsrc, int *d_levels, int use_graph, int *h_nlevels,
+                      bgb_stream_t stream) {
   clear_error();
   int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_levels, h_nlevels);
   if (st) return st;
   BGB_REQUIRE(h_sources && nsrc >= 1 && nsrc <= 64, BGB_ERR_ARG, "need 1..64 sources");
   for (int i = 0; i < nsrc; ++i)
     BGB_REQUIRE(h_sources[i] >= 0 && h_sources[i] < n, BGB_ERR_ARG, "source %d out of range", h_sources[i]);
+  BGB_REQUIRE((d_At == nullptr) == (d_rowptrT == nullptr) && (d_At == nullptr) == (d_colindT == nullptr),
+              BGB_ERR_ARG, "pass all of At, rowptrT, colindT, or none");
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (T) {
-    case 4: return bfs_multi_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    case 8: return bfs_multi_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    case 16: return bfs_multi_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    default: return bfs_multi_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    case 4: return bfs_multi_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    case 8: return bfs_multi_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    case 16: return bfs_multi_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    default: return bfs_multi_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
   }
 }
 

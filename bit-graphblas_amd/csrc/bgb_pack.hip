@@ -65,20 +65,23 @@ __global__ void validate_cols_kernel(const int *__restrict__ colind, long long n
   if (j < 0 || j >= ncols || (k > 0 && !row_start[k] && colind[k - 1] >= j)) atomicOr(bad, 8);
 }
 
-// One wave per CSR row; lanes stride over the row's nonzeros.
+// One wave per CSR row (grid-stride over rows: 64 work-items per row would
+// pass the dispatch's 32-bit grid size beyond 2^26 rows); lanes stride over
+// the row's nonzeros.
 __global__ void make_keys_kernel(const int *__restrict__ rowptr, const int *__restrict__ colind,
                                  int nrows, KeyLayout kl, uint64_t *__restrict__ keys) {
-  const long long row = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (row >= nrows) return;
   const int lane = threadIdx.x & 63;
   const int mask = (1 << kl.lg) - 1;
-  const uint64_t I = (uint64_t)(row >> kl.lg);
-  const uint64_t r = (uint64_t)(row & mask);
-  const uint64_t hi = (I << (kl.bitsJ + 2 * kl.lg)) | (r << kl.lg);
-  const int s = rowptr[row], e = rowptr[row + 1];
-  for (int k = s + lane; k < e; k += 64) {
-    const uint32_t j = (uint32_t)colind[k];
-    keys[k] = hi | ((uint64_t)(j >> kl.lg) << (2 * kl.lg)) | (uint64_t)(j & mask);
+  const long long waves = (long long)gridDim.x * (kBlock / 64);
+  for (long long row = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); row < nrows; row += waves) {
+    const uint64_t I = (uint64_t)(row >> kl.lg);
+    const uint64_t r = (uint64_t)(row & mask);
+    const uint64_t hi = (I << (kl.bitsJ + 2 * kl.lg)) | (r << kl.lg);
+    const int s = rowptr[row], e = rowptr[row + 1];
+    for (int k = s + lane; k < e; k += 64) {
+      const uint32_t j = (uint32_t)colind[k];
+      keys[k] = hi | ((uint64_t)(j >> kl.lg) << (2 * kl.lg)) | (uint64_t)(j & mask);
+    }
   }
 }
 
@@ -158,19 +161,21 @@ __global__ void tile_fill_kernel(const uint64_t *__restrict__ keys, const int *_
 // T = 64: one thread per (tile, row) word. The tile's nonzeros are sorted by
 // (row, column), so the thread ORs the bits of its row's run; 64 words of 8 B
 // per tile would not fit one thread's registers.
+// Grid-stride: 64 words per tile pass the 32-bit grid size beyond 2^26 tiles.
 __global__ void tile_fill64_kernel(const uint64_t *__restrict__ keys, const int *__restrict__ head_pos,
                                    long long nwords, uint64_t *__restrict__ val) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nwords) return;
-  const long long tile = t >> 6;
-  const int r = (int)(t & 63);
-  const int s = head_pos[tile], e = head_pos[tile + 1];
-  uint64_t w = 0;
-  for (int k = s; k < e; ++k) {
-    const uint32_t rc = (uint32_t)(keys[k] & 0xFFFu);  // 6 row bits, 6 column bits
-    if ((int)(rc >> 6) == r) w |= 1ull << (63 - (rc & 63));
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nwords; t += stride) {
+    const long long tile = t >> 6;
+    const int r = (int)(t & 63);
+    const int s = head_pos[tile], e = head_pos[tile + 1];
+    uint64_t w = 0;
+    for (int k = s; k < e; ++k) {
+      const uint32_t rc = (uint32_t)(keys[k] & 0xFFFu);  // 6 row bits, 6 column bits
+      if ((int)(rc >> 6) == r) w |= 1ull << (63 - (rc & 63));
+    }
+    val[t] = w;
   }
-  val[t] = w;
 }
 
 template <int T>
@@ -243,7 +248,7 @@ int sorted_tiles(int T, int nrows, int ncols, int nnz, const int *rowptr, const 
   BGB_HIP_TRY(flags.alloc((size_t)nnz + 1, s));
   BGB_HIP_TRY(excl.alloc(sizeof(long long) * ((size_t)nnz + 1), s));
   if (nrows > 0)
-    make_keys_kernel<<<grid_for(nrows, kBlock / 64), kBlock, 0, s>>>(rowptr, colind, nrows, kl,
+    make_keys_kernel<<<capped_blocks((long long)nrows * 64, kBlock), kBlock, 0, s>>>(rowptr, colind, nrows, kl,
                                                                      raw.as<uint64_t>());
   BGB_HIP_TRY(hipGetLastError());
   if (nnz > 0) BGB_HIP_TRY(radix_sort_u64(raw.as<uint64_t>(), keys.as<uint64_t>(), nnz, kl.end_bit(), s));
@@ -319,7 +324,7 @@ int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
     case 16: tile_fill_kernel<16><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
     case 32: tile_fill_kernel<32><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
     default:
-      tile_fill64_kernel<<<grid_for(64LL * nblocks), kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(),
+      tile_fill64_kernel<<<capped_blocks(64LL * nblocks, kBlock), kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(),
                                                                      64LL * nblocks, static_cast<uint64_t *>(d_bsrVal));
       break;
   }

@@ -217,9 +217,12 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
                     int nworkgroups, bgb_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
- * BFS on a symmetric B2SR matrix (no reference counterpart; the reference's *
- * building block is the masked BMV above). n = number of vertices,         *
- * nblockrows = ceil(n/T). Levels are int32 hop counts, -1 = unreached.     *
+ * BFS over a square B2SR matrix (no reference counterpart; the reference's  *
+ * building block is the masked BMV above). Vertex v is reached from the     *
+ * vertices u with A[v][u] = 1 (its row: pull), so a symmetric A gives the   *
+ * undirected BFS and the transpose of a directed graph's matrix its forward *
+ * BFS. n = number of vertices, nblockrows = ceil(n/T). Levels are int32 hop *
+ * counts, -1 = unreached.                                                   *
  * ------------------------------------------------------------------------ */
 
 /* Up to 64 sources at once: one uint64 frontier word per vertex, one pull
@@ -229,6 +232,18 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
 int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
                   int nblockrows, int nblocks, int n, const int *h_sources, int nsrc,
                   int *d_levels, int use_graph, int *h_nlevels, bgb_stream_t stream);
+
+/* bgb_bfs_multi with direction switching: d_At / d_rowptrT / d_colindT are
+ * the B2SR of A^T (pass A's own arrays when A is symmetric; all NULL = pull
+ * only, as bgb_bfs_multi). Levels whose frontier is cheap to push (the
+ * first, and those whose frontier vertices' A^T block rows hold fewer than
+ * nblocks/64 tiles in total) push along the rows of A^T with atomics instead
+ * of pulling over every tile; the others pull. Same results as
+ * bgb_bfs_multi. */
+int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
+                      int nblockrows, int nblocks, const void *d_At, const int *d_rowptrT,
+                      const int *d_colindT, int n, const int *h_sources, int nsrc, int *d_levels,
+                      int use_graph, int *h_nlevels, bgb_stream_t stream);
 
 /* One source: one masked bin_bin_bin BMV per level (frontier & ~visited),
  * the per-source loop the batched BFS is compared with. Synchronises once

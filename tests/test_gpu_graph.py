@@ -47,11 +47,12 @@ def test_bfs_multi_random(orc, dev, T):
     d2 = rng.integers(1500, 2900, 2500)
     csr = graph.from_coo(n, n, np.concatenate([s1, s2]), np.concatenate([d1, d2]))
     A = B.B2SR.from_csr(csr, T, dev)
-    for nsrc, use_graph in ((64, True), (64, False), (5, True), (1, False)):
-        log(f"[case] bfs_multi T={T} nsrc={nsrc} graph={use_graph}")
+    for nsrc, use_graph, push in ((64, True, None), (64, False, None), (5, True, None), (1, False, None),
+                                  (64, True, A), (64, False, A), (3, True, A)):
+        log(f"[case] bfs_multi T={T} nsrc={nsrc} graph={use_graph} push={push is not None}")
         src = rng.integers(0, n, nsrc)
         src[0] = 2950  # an isolated vertex
-        lev, nl = B.bfs_multi(A, src, use_graph)
+        lev, nl = B.bfs_multi(A, src, use_graph, push=push)
         want = oracle_levels(orc, csr, src)
         np.testing.assert_array_equal(lev.cpu().numpy(), want)
         assert nl == want.max() + 1
@@ -84,6 +85,10 @@ def test_bfs_rmat_batched_equals_plain_and_oracle(orc, dev):
         A = B.B2SR.from_csr(csr, T, dev)
         lev, nl = B.bfs_multi(A, src)
         np.testing.assert_array_equal(lev.cpu().numpy(), want)
+        # direction switching (symmetric: A is its own transpose) gives the same levels
+        levd, nld = B.bfs_multi(A, src, push=A)
+        np.testing.assert_array_equal(levd.cpu().numpy(), want)
+        assert nld == nl
         if T == 8:
             for i in (0, 31, 63):
                 pl, _ = B.bfs_plain(A, int(src[i]))

@@ -201,3 +201,8 @@ def test_directed_bfs_pulls_over_the_transpose(orc, dev, T):
     lev_rev, _ = B.bfs_multi(A, src)
     want_rev = np.stack([orc.bfs(cp, ri, int(s))[0] for s in src])
     np.testing.assert_array_equal(lev_rev.cpu().numpy(), want_rev)
+    # direction switching pushes along the rows of the pulled matrix's transpose
+    lev_d, _ = B.bfs_multi(At, src, push=A)
+    np.testing.assert_array_equal(lev_d.cpu().numpy(), want)
+    lev_rd, _ = B.bfs_multi(A, src, push=At)
+    np.testing.assert_array_equal(lev_rd.cpu().numpy(), want_rev)
