@@ -48,7 +48,7 @@ namespace {
 constexpr int kBfsWaves = 4;
 constexpr int kBfsRun = 8;
 constexpr int kPlanes = 16;  // level < 2^16
-constexpr int kPushDiv = 64;    // push while the next frontier's push cost < nblocks / kPushDiv tiles
+constexpr int kPushDiv = 16;    // push while the next frontier's push cost < nblocks / kPushDiv tiles
 constexpr int kCounters = 256;  // push-cost counters, spread to avoid one hot address
 constexpr int kPushSeg = 2048;  // push: frontier rows of longer block rows are split into segments
 constexpr int kBigCap = 1 << 16;  // push: (block row, row) entries of the long-row list
@@ -579,7 +579,7 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   }
   const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
   const unsigned fblocks = nblk(nchunks * T);
-  const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 1 << 16);
+  const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 8192);
   const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
   const long long zbytes = wb * nwords + 16;
   auto level_pair = [&]() {

@@ -237,7 +237,7 @@ int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_coli
  * the B2SR of A^T (pass A's own arrays when A is symmetric; all NULL = pull
  * only, as bgb_bfs_multi). Levels whose frontier is cheap to push (the
  * first, and those whose frontier vertices' A^T block rows hold fewer than
- * nblocks/64 tiles in total) push along the rows of A^T with atomics instead
+ * nblocks/16 tiles in total) push along the rows of A^T with atomics instead
  * of pulling over every tile; the others pull. Same results as
  * bgb_bfs_multi. */
 int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_colind,
