@@ -35,7 +35,9 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 constexpr int kK = 16;        // right-hand sides
 constexpr int kChunk = 256;   // tiles per wave
 constexpr int kWaves = 4;     // waves per workgroup
-constexpr int kUnroll = 4;    // tiles per pipelined group
+// tiles per pipelined group (X gathers in flight per wave)
+template <int T>
+constexpr int kGroup = 8;
 
 // 4 bits (bit 3 = first column) -> four FP16 values 0.0 / 1.0.
 __device__ __forceinline__ half4_t nib_to_h4(uint32_t nib) {
@@ -68,49 +70,73 @@ __device__ __forceinline__ void k16_flush(const K16Graph &G, int I, int lane, co
     }
 }
 
+// Wave-uniform index load through the scalar data cache (s_load): block
+// columns and row ends are the same for every lane, and SMEM loads are counted
+// apart from the vector loads, so waiting for one does not drain the other.
+typedef __attribute__((address_space(4))) const int const_int_t;
+__device__ __forceinline__ int sload(const int *p, long long i) {
+  return ((const const_int_t *)p)[i];  // address-space cast: C-style only
+}
+
+// Lane's 8 bytes of the X block at block column J (MH 512-byte halves), or 0
+// when `use` is false. The buffer resource spans exactly the block, and a
+// masked lane's offset lies past its end: the hardware's range check returns
+// zero for it with no memory access, so the gather needs no branch.
+template <int MH>
+__device__ __forceinline__ uint2 x_gather(const uint2 *xb, int J, int h, int lane, bool use) {
+  constexpr int kBytes = MH * 512;
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint2 *>(xb + (long long)J * MH * 64), (short)0, kBytes, 0x00020000);
+  return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, use ? h * 512 + lane * 8 : kBytes, 0, 0));
+}
+
 // Processes tiles [k0, k1) of one graph; returns the row left open (or nbr).
+//
+// Groups of kGroup<T> tiles: the X gathers of group g are issued, then the row
+// words of group g+1 (block columns through SMEM), then the MFMAs of group g
+// wait for the gathers alone. Every load is issued unconditionally (indices
+// past the chunk are clamped to its last tile and their words zeroed), so the
+// compiler can count the loads in flight instead of draining them.
 template <int T>
 __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane, float *carry) {
-  constexpr int MH = T / 16;
+  constexpr int MH = T / 16, U = kGroup<T>;
   using W = typename Tile<T>::W;
-  const int *__restrict__ rowptr = G.rowptr;
   const W *__restrict__ A = static_cast<const W *>(G.val);
   const int m = lane & 15, kq = lane >> 4;
-  int I = wave_find_row(rowptr, 0, G.nbr - 1, k0, lane);
+  int I = __builtin_amdgcn_readfirstlane(wave_find_row(G.rowptr, 0, G.nbr - 1, k0, lane));
   float4_t D[MH], Z[MH];
 #pragma unroll
   for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh] = float4_t{0.f, 0.f, 0.f, 0.f};
   if (k0 == 0)
     for (int r = 0; r < I; ++r) k16_flush<T>(G, r, lane, Z);
-  int e = rowptr[I + 1];
-  // Groups of kUnroll tiles, software-pipelined: the block columns and row
-  // words of group g+1 are in flight while group g loads its X blocks and runs
-  // its MFMAs (one exposed memory latency per group instead of two).
-  uint32_t w[kUnroll][MH], wn[kUnroll][MH];
-  int J[kUnroll], Jn[kUnroll];
-  auto load_tiles = [&](long long p0, uint32_t (&ww)[kUnroll][MH], int (&jj)[kUnroll]) {
+  // e: end of row I; e2: end of row I + 1 (prefetched, read at the next row end)
+  int e = sload(G.rowptr, I + 1);
+  int e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
+  const long long last = k1 - 1;
+  uint32_t w[U][MH], wn[U][MH];
+  int J[U], Jn[U];
+  auto load_tiles = [&](long long p0, uint32_t (&ww)[U][MH], int (&jj)[U]) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const long long p = p0 + u;
-      jj[u] = 0;
+    for (int u = 0; u < U; ++u) {
+      const long long p = p0 + u < last ? p0 + u : last;
+      jj[u] = sload(G.colind, p);
 #pragma unroll
-      for (int h = 0; h < MH; ++h) ww[u][h] = 0;
-      if (p < k1) {
-        jj[u] = G.colind[p];
-#pragma unroll
-        for (int h = 0; h < MH; ++h) ww[u][h] = A[p * T + h * 16 + m];
-      }
+      for (int h = 0; h < MH; ++h) ww[u][h] = A[p * T + h * 16 + m];
     }
   };
   load_tiles(k0, w, J);
-  for (long long p0 = k0; p0 < k1; p0 += kUnroll) {
+  for (long long p0 = k0; p0 < k1; p0 += U) {
     // X rows of a 4-column group only matter where the tile has a set bit in
     // those columns (the A operand is 0 elsewhere): the tile's column mask is
     // the OR of its row words, which sit in one 16-lane DPP row (lane m holds
-    // row m), so lanes of groups with no set bit load nothing.
-    uint2 xv[kUnroll][MH];
+    // row m), so lanes of groups with no set bit read nothing. Tiles past the
+    // chunk get an empty mask, hence zero X and zero A.
+    uint2 xv[U][MH];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
+      const bool in = p0 + u < k1;
+#pragma unroll
+      for (int h = 0; h < MH; ++h) w[u][h] = in ? w[u][h] : 0u;
       uint32_t cols = 0;
 #pragma unroll
       for (int h = 0; h < MH; ++h) cols |= w[u][h];
@@ -118,46 +144,42 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
 #pragma unroll
       for (int h = 0; h < MH; ++h) {
         const uint32_t grp = (cols >> (T - 4 - (h * 16 + 4 * kq))) & 0xFu;
-        xv[u][h] = (p0 + u < k1 && grp) ? G.xb[((long long)J[u] * MH + h) * 64 + lane] : make_uint2(0u, 0u);
+        xv[u][h] = x_gather<MH>(G.xb, J[u], h, lane, grp != 0);
       }
     }
-    const bool more = p0 + kUnroll < k1;  // wave-uniform
-    if (more) load_tiles(p0 + kUnroll, wn, Jn);
+    load_tiles(p0 + U, wn, Jn);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const long long p = p0 + u;
-      if (p < k1) {
+    for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int kh = 0; kh < MH; ++kh) {
-          const half4_t b = __builtin_bit_cast(half4_t, xv[u][kh]);
+      for (int kh = 0; kh < MH; ++kh) {
+        const half4_t b = __builtin_bit_cast(half4_t, xv[u][kh]);
 #pragma unroll
-          for (int mh = 0; mh < MH; ++mh) {
-            const uint32_t nib = (w[u][mh] >> (T - 4 - (kh * 16 + 4 * kq))) & 0xFu;
-            D[mh] = __builtin_amdgcn_mfma_f32_16x16x16f16(nib_to_h4(nib), b, D[mh], 0, 0, 0);
-          }
+        for (int mh = 0; mh < MH; ++mh) {
+          const uint32_t nib = (w[u][mh] >> (T - 4 - (kh * 16 + 4 * kq))) & 0xFu;
+          D[mh] = __builtin_amdgcn_mfma_f32_16x16x16f16(nib_to_h4(nib), b, D[mh], 0, 0, 0);
         }
-        const int pos = (int)p + 1;
-        if (pos == e) {
-          k16_flush<T>(G, I, lane, D);
+      }
+      const long long pos = p0 + u + 1;
+      if (pos <= k1 && pos == e) {  // wave-uniform (scalar) test
+        k16_flush<T>(G, I, lane, D);
 #pragma unroll
-          for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh];
+        for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh];
+        ++I;
+        e = e2;
+        e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
+        while (e == pos) {  // empty rows
+          k16_flush<T>(G, I, lane, Z);
           ++I;
-          e = I < G.nbr ? rowptr[I + 1] : INT_MAX;
-          while (e == pos) {
-            k16_flush<T>(G, I, lane, Z);
-            ++I;
-            e = I < G.nbr ? rowptr[I + 1] : INT_MAX;
-          }
+          e = e2;
+          e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
         }
       }
     }
-    if (more) {
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        J[u] = Jn[u];
+    for (int u = 0; u < U; ++u) {
+      J[u] = Jn[u];
 #pragma unroll
-        for (int h = 0; h < MH; ++h) w[u][h] = wn[u][h];
-      }
+      for (int h = 0; h < MH; ++h) w[u][h] = wn[u][h];
     }
   }
   if (I < G.nbr) {
@@ -180,7 +202,7 @@ struct K16Args {
 template <int T>
 __global__ __launch_bounds__(64 * kWaves) void k16_kernel(K16Args a) {
   const int lane = threadIdx.x & 63;
-  const int chunk = xcd_remap(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6));
   if (chunk >= a.nchunks) return;
   const long long k0 = (long long)chunk * kChunk;
   const long long k1 = k0 + kChunk < a.g.nblocks ? k0 + kChunk : a.g.nblocks;
@@ -239,7 +261,12 @@ __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArg
     const unsigned long long base = s_ticket;
     __syncthreads();  // every thread has read the ticket before the next one is taken
     if (base >= (unsigned long long)a.total) break;  // uniform per workgroup
-    const unsigned long long c = base + (unsigned long long)wave;
+    // the chunk and its graph are wave-uniform: kept in scalar registers, so the
+    // graph's pointers feed the buffer resources of the X gathers directly
+    const unsigned long long c =
+        ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32 |
+         (unsigned)__builtin_amdgcn_readfirstlane((int)base)) +
+        (unsigned)__builtin_amdgcn_readfirstlane(wave);
     if (c < (unsigned long long)a.total) {  // uniform per wave
       int lo = 0, hi = a.ngraphs - 1;  // graph with chunk_start[g] <= c < chunk_start[g+1]
       while (lo < hi) {
@@ -247,6 +274,7 @@ __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArg
         if (a.chunk_start[mid] <= (long long)c) lo = mid;
         else hi = mid - 1;
       }
+      lo = __builtin_amdgcn_readfirstlane(lo);
       const K16Graph G = a.graphs[lo];
       const long long k0 = ((long long)c - a.chunk_start[lo]) * kChunk;
       const long long k1 = k0 + kChunk < G.nblocks ? k0 + kChunk : G.nblocks;

@@ -182,6 +182,9 @@ int bgb_bmv_bin_bin_bin_masked(int T, const void *d_A, const void *d_B, void *d_
  * K = 16 right-hand sides on MFMA (no reference counterpart; the reference's *
  * bin_full_full takes one FP32 vector). Tile dims 16 and 32. X is FP16      *
  * (raw uint16 bits), 0/1 tiles are exact in FP16, accumulation is FP32.     *
+ * X must be finite: a product runs over every column of a 4-column group    *
+ * that holds a set bit, so an Inf/NaN there becomes 0*Inf in the tile's     *
+ * other rows.                                                                *
  * ------------------------------------------------------------------------ */
 
 /* Permutes X (row-major FP16 [n][16]) into the MFMA operand order used by

@@ -85,10 +85,11 @@ def test_k16_persistent_batch(orc, dev):
 
 
 @pytest.mark.timeout(600)
-def test_k16_batch_beyond_2e32_work_items(orc, dev):
-    """A batch whose carry fix-up covers more than 2^32 values (three R-MAT scale-22
-    graphs at T=16: ~8.6 M chunks x 512): the fix-up walks its range with a capped
-    grid, since a dispatch's work-item count is 32 bits."""
+def test_k16_batch_fixup_grid_stride(orc, dev):
+    """A batch whose carry fix-up outgrows its capped grid (three R-MAT scale-22
+    graphs at T=16, 256-tile chunks, 512 carry values each): the fix-up grid is
+    capped at 2^20 workgroups of 256 (a dispatch's work-item count is 32 bits),
+    so every thread walks the range in more than one stride."""
     from bgb import b2sr as B
     from bgb import graph
     mats, xbs, refs = [], [], []
@@ -100,8 +101,8 @@ def test_k16_batch_beyond_2e32_work_items(orc, dev):
         mats.append(A)
         xbs.append(B.k16_pack_rhs(X, A.nblockrows * 16))
         refs.append((csr, X64))
-        chunks += -(-A.nblocks // 32)
-    assert chunks * 512 > 2 ** 32
+        chunks += -(-A.nblocks // 256)
+    assert chunks * 512 > (1 << 20) * 256
     log(f"[case] k16 batch of 3 R-MAT 22 graphs: {chunks} chunks")
     batch = B.K16Batch(mats, xbs)
     batch.run()
