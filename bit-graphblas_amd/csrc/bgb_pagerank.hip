@@ -26,6 +26,9 @@ namespace bgb {
 namespace {
 
 constexpr int kPrBlock = 256;
+// Update / init grids: at most this many workgroups (grid-stride loops), so
+// the single finishing block folds at most this many partials per sum.
+constexpr int kPrMaxBlocks = 1024;
 
 // float -> bfloat16 with round-to-nearest-even on the upper 16 bits.
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
@@ -46,21 +49,23 @@ __device__ __forceinline__ Acc block_sum(Acc v, Acc *sh) {
   return t;
 }
 
-// x from pr, dangling partials; used at start and after each update.
+// x from pr, dangling partials; used at start. Like the update kernel, a
+// grid-stride loop over a grid of at most kPrMaxBlocks workgroups, so the
+// finishing block folds few partials.
 template <class Acc, bool BF>
 __global__ __launch_bounds__(kPrBlock) void pr_init_kernel(float *pr, const int *deg, int n, long long npad,
                                                            void *x, Acc *part_dang) {
   __shared__ Acc sh[kPrBlock / 64];
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   Acc dang = 0;
-  if (v < npad) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < npad;
+       v += (long long)gridDim.x * blockDim.x) {
     float p = v < n ? 1.0f / (float)n : 0.0f;
     if (v < n) pr[v] = p;
     const int dg = v < n ? deg[v] : 0;
     const float xv = dg > 0 ? p / (float)dg : 0.0f;
     if (BF) static_cast<uint16_t *>(x)[v] = f32_to_bf16(xv);
     else static_cast<float *>(x)[v] = xv;
-    if (v < n && dg == 0) dang = (Acc)p;
+    if (v < n && dg == 0) dang += (Acc)p;
   }
   const Acc t = block_sum<Acc>(dang, sh);
   if (threadIdx.x == 0) part_dang[blockIdx.x] = t;
@@ -72,20 +77,21 @@ __global__ __launch_bounds__(kPrBlock) void pr_update_kernel(const float *y, con
                                                              const Acc *dang_total, void *x, Acc *part_res,
                                                              Acc *part_dang) {
   __shared__ Acc sh[kPrBlock / 64];
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Acc tele = (Acc)(1.0f - damping) / (Acc)n;
+  const Acc dn = *dang_total / (Acc)n;
   Acc res = 0, dang = 0;
-  if (v < npad) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < npad;
+       v += (long long)gridDim.x * blockDim.x) {
     float xv = 0.0f;
     if (v < n) {
-      const Acc tele = (Acc)(1.0f - damping) / (Acc)n;
-      const Acc pn = tele + (Acc)damping * ((Acc)y[v] + *dang_total / (Acc)n);
+      const Acc pn = tele + (Acc)damping * ((Acc)y[v] + dn);
       const Acc po = (Acc)pr[v];
-      res = pn > po ? pn - po : po - pn;
+      res += pn > po ? pn - po : po - pn;
       const float p = (float)pn;
       pr_next[v] = p;
       const int dg = deg[v];
       xv = dg > 0 ? p / (float)dg : 0.0f;
-      if (dg == 0) dang = (Acc)p;
+      if (dg == 0) dang += (Acc)p;
     }
     if (BF) static_cast<uint16_t *>(x)[v] = f32_to_bf16(xv);
     else static_cast<float *>(x)[v] = xv;
@@ -104,6 +110,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_finish_kernel(const Acc *part_res
                                                              double *res_out, Acc *dang_total) {
   __shared__ Acc sh[kPrBlock / 64];
   Acc r = 0, d = 0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < nparts; i += kPrBlock) {
     if (part_res) r += part_res[i];
     d += part_dang[i];
@@ -118,10 +125,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_finish_kernel(const Acc *part_res
 
 // deg(v) = popcount of row v over its block row's tiles.
 template <int T>
-__global__ void row_degree_kernel(const void *A, const int *rowptr, int nbr, int *deg) {
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= (long long)nbr * T) return;
-  const int I = (int)(v / T), r = (int)(v % T);
+__device__ __forceinline__ int row_degree(const void *A, const int *rowptr, int I, int r) {
   int d = 0;
   for (int k = rowptr[I]; k < rowptr[I + 1]; ++k) {
     if constexpr (T == 64) {
@@ -134,14 +138,22 @@ __global__ void row_degree_kernel(const void *A, const int *rowptr, int nbr, int
       d += __popc(w);
     }
   }
-  deg[v] = d;
+  return d;
+}
+
+template <int T>
+__global__ void row_degree_kernel(const void *A, const int *rowptr, int nbr, int *deg) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < (long long)nbr * T;
+       v += (long long)gridDim.x * blockDim.x)
+    deg[v] = row_degree<T>(A, rowptr, (int)(v / T), (int)(v % T));
 }
 
 template <class Acc, bool BF>
 int pagerank_impl(int T, const void *A, const int *rowptr, const int *colind, int nbr, int nblocks, int n,
                   float damping, int iters, float *pr_out, double *h_res, hipStream_t s) {
   const long long npad = (long long)nbr * T;
-  const unsigned blocks = (unsigned)ceil_div(npad, kPrBlock);
+  const unsigned blocks = (unsigned)(ceil_div(npad, kPrBlock) < kPrMaxBlocks ? ceil_div(npad, kPrBlock) : kPrMaxBlocks);
+  const unsigned deg_blocks = capped_blocks(npad, kPrBlock);
   DeviceBuffer deg, x, y, pr2, pres, pdang, dtot, res;
   BGB_HIP_TRY(deg.alloc(sizeof(int) * npad, s));
   BGB_HIP_TRY(x.alloc((BF ? 2 : 4) * npad + 16, s));
@@ -152,11 +164,11 @@ int pagerank_impl(int T, const void *A, const int *rowptr, const int *colind, in
   BGB_HIP_TRY(dtot.alloc(sizeof(Acc), s));
   BGB_HIP_TRY(res.alloc(sizeof(double) * (iters > 0 ? iters : 1), s));
   switch (T) {
-    case 4: row_degree_kernel<4><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
-    case 8: row_degree_kernel<8><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
-    case 16: row_degree_kernel<16><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
-    case 32: row_degree_kernel<32><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
-    default: row_degree_kernel<64><<<blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 4: row_degree_kernel<4><<<deg_blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 8: row_degree_kernel<8><<<deg_blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 16: row_degree_kernel<16><<<deg_blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    case 32: row_degree_kernel<32><<<deg_blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
+    default: row_degree_kernel<64><<<deg_blocks, kPrBlock, 0, s>>>(A, rowptr, nbr, deg.as<int>()); break;
   }
   pr_init_kernel<Acc, BF><<<blocks, kPrBlock, 0, s>>>(pr_out, deg.as<int>(), n, npad, x.ptr, pdang.as<Acc>());
   pr_finish_kernel<Acc><<<1, kPrBlock, 0, s>>>(nullptr, pdang.as<Acc>(), (int)blocks, nullptr, dtot.as<Acc>());
