@@ -119,6 +119,31 @@ def test_pagerank_grid_modes(orc, dev, T):
             np.testing.assert_allclose(res[:10], rres[:10], rtol=1e-3)
 
 
+def test_pagerank_grid_stride(orc, dev):
+    """More vertices than one pass of the capped update grid covers (1024
+    workgroups x 256), R-MAT with isolated vertices so the dangling sum is
+    non-zero: ranks and residuals match the oracle, and a second run matches
+    the first to FP32 rounding (the partial sums are folded in index order;
+    the BMV adds the FP32 carries of rows spanning more than 64 chunks with
+    atomics, whose order can move a sum by an ulp)."""
+    from bgb import b2sr as B
+    from bgb import graph
+    csr = graph.rmat(19, 8, seed=9)
+    assert csr.nrows > 1024 * 256 and (csr.degrees() == 0).any()
+    ref, rres = orc.pagerank(csr.rowptr, csr.colind, 0.85, 20)
+    A = B.B2SR.from_csr(csr, 8, dev)
+    for mode in ("fp32", "fp64sum"):
+        log(f"[case] pagerank rmat19 mode={mode}")
+        pr, res = B.pagerank(A, 0.85, 20, mode)
+        assert pr_errors(pr.cpu().numpy(), ref) < PR_RTOL[mode]
+        # FP32 rounding moves an FP32 replay of the iteration's residuals by
+        # < 2e-5 relative over the first 8 steps (numpy check of this graph)
+        np.testing.assert_allclose(res[:8], rres[:8], rtol=1e-3)
+        pr2, res2 = B.pagerank(A, 0.85, 20, mode)
+        np.testing.assert_allclose(pr2.cpu().numpy(), pr.cpu().numpy(), rtol=1e-5, atol=0)
+        np.testing.assert_allclose(res2[:8], res[:8], rtol=1e-3)
+
+
 def test_pagerank_rmat_with_dangling(orc, dev):
     from bgb import b2sr as B
     from bgb import graph
