@@ -71,6 +71,9 @@ constexpr int kWavesPerBlock = 4;
 int g_run_length[5] = {8, 8, 8, 8, 8};
 // Packed-x staging per tile dim: 0 automatic, 1 never, 2 whenever it fits.
 int g_x_staging[5] = {0, 0, 0, 0, 0};
+// Short-row path (T <= 8): 0 automatic (every block row at most kShortRowMax
+// tiles), 1 never, 2 always.
+int g_short_rows = 0;
 
 template <int OP>
 struct AccOf {
@@ -313,7 +316,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         for (int i = 0; i < R; ++i) {
           const Word<T> w = row_word<T>(F[s], i);
 #pragma unroll
-          for (int c = 0; c < T; ++c) acc[i] += ((w >> (T - 1 - c)) & 1u) ? XS[s][c] : 0.0f;
+          for (int c = 0; c < T; ++c) acc[i] += bit_select(w, T - 1 - c, XS[s][c]);
         }
       } else if constexpr (OP == OP_FULL_F32) {
         const float *__restrict__ x = static_cast<const float *>(a.B) + (long long)J[s] * T;
@@ -617,6 +620,116 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
   }
 }
 
+// Short-row path (T <= 8, every block row at most kShortRowMax tiles): one
+// lane per block row walks the row's tiles in batches of U tiles with
+// unconditional loads (indices clamped into the row, contributions of the
+// padding masked), then writes the row once -- no chunk partition, row
+// window, segmented scan or carries. For regular matrices (a 2-D grid has 2-3
+// tiles per block row) the chunk walk would otherwise spend most of its time
+// on row ends: a run of 8 tiles crosses ~3 of them.
+constexpr int kShortRowMax = 32;
+
+// U: tiles per batch of loads, picked from the longest block row (a row of
+// at most 8 tiles is one batch with the fewest padding slots: on the 4096^2
+// grid, 5 tiles per block row, batches of 5 run T = 8 in 67 us, of 6 in 72,
+// of 4 -- two batches -- in 83).
+template <int T, int OP, int U>
+__global__ __launch_bounds__(256) void bmv_rows_kernel(BmvArgs a) {
+  static_assert(Tile<T>::L == 1, "one lane per block row needs T <= 8");
+  constexpr int R = Tile<T>::R, NW = Tile<T>::NW;
+  using Acc = typename AccOf<OP>::type;
+  using W = typename Tile<T>::W;
+  // consecutive block rows on one XCD: the x segments a row gathers (for a
+  // grid, block columns I and I +- one grid row) stay in that XCD's L2
+  for (long long I = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; I < a.nbr;
+       I += (long long)gridDim.x * blockDim.x) {
+    const int s = a.rowptr[I], e = a.rowptr[I + 1];
+    Acc acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i] = 0;
+    for (int k = s; k < e; k += U) {
+      uint32_t F[U][NW];
+      int J[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long p = k + u < e ? k + u : e - 1;
+        J[u] = a.colind[p];
+        load_frag<T>(a.A, p, 0, F[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k + u >= e) {
+#pragma unroll
+          for (int w = 0; w < NW; ++w) F[u][w] = 0u;  // padding: no set bits
+        }
+      if constexpr (is_full(OP)) {
+        float XS[U][T];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if constexpr (OP == OP_FULL_F32) {
+            const float4 *xp = reinterpret_cast<const float4 *>(static_cast<const float *>(a.B) + (long long)J[u] * T);
+#pragma unroll
+            for (int v = 0; v < T / 4; ++v) {
+              const float4 f = xp[v];
+              XS[u][4 * v] = f.x;
+              XS[u][4 * v + 1] = f.y;
+              XS[u][4 * v + 2] = f.z;
+              XS[u][4 * v + 3] = f.w;
+            }
+          } else {
+            const uint2 *xp = reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.B) + (long long)J[u] * T);
+#pragma unroll
+            for (int v = 0; v < T / 4; ++v) {
+              const uint2 h = xp[v];
+              XS[u][4 * v] = bf16_to_f32((uint16_t)(h.x & 0xFFFFu));
+              XS[u][4 * v + 1] = bf16_to_f32((uint16_t)(h.x >> 16));
+              XS[u][4 * v + 2] = bf16_to_f32((uint16_t)(h.y & 0xFFFFu));
+              XS[u][4 * v + 3] = bf16_to_f32((uint16_t)(h.y >> 16));
+            }
+          }
+        }
+        // tiles in order, ascending columns within a tile, set bits only:
+        // the CSR summation order. (Skipping slots that are padding in every
+        // lane behind a wave-uniform branch let the compiler sink each slot's
+        // loads into its branch, one round trip per tile: slower.)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            const Word<T> w = row_word<T>(F[u], i);
+#pragma unroll
+            for (int c = 0; c < T; ++c) acc[i] += bit_select(w, T - 1 - c, XS[u][c]);
+          }
+      } else {
+        Word<T> X[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) X[u] = (Word<T>) static_cast<const W *>(a.B)[J[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < R; ++i) acc[i] += bit_count(row_word<T>(F[u], i) & X[u]);
+      }
+    }
+    flush_row<T, OP>(a, (int)I, 0, acc);
+  }
+}
+
+// Most tiles in one block row (device-wide max of rowptr[I+1] - rowptr[I]).
+__global__ void max_row_tiles_kernel(const int *rowptr, int nbr, int *out) {
+  int m = 0;
+  for (long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x; I < nbr;
+       I += (long long)gridDim.x * blockDim.x) {
+    const int d = rowptr[I + 1] - rowptr[I];
+    m = d > m ? d : m;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int t = __shfl_xor(m, o);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
 template <class V>
 __global__ void fill_kernel(V *p, long long n, V v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -684,6 +797,26 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
 
 template <int T, int OP>
 int launch_bmv(BmvArgs a, hipStream_t s) {
+  if constexpr (Tile<T>::L == 1) {
+    int mr = -1;  // most tiles in a block row; -1: unknown
+    if (g_short_rows == 2) {
+      mr = 0;
+    } else if (g_short_rows == 0) {
+      const int st = max_row_tiles(s, a.rowptr, a.nbr, a.nblocks, &mr);
+      if (st) return st;
+    }
+    if (mr >= 0 && mr <= kShortRowMax) {
+      const unsigned grid = capped_blocks(a.nbr, 256);
+      switch (mr <= 4 ? 4 : mr <= 6 ? mr : 8) {
+        case 4: bmv_rows_kernel<T, OP, 4><<<grid, 256, 0, s>>>(a); break;
+        case 5: bmv_rows_kernel<T, OP, 5><<<grid, 256, 0, s>>>(a); break;
+        case 6: bmv_rows_kernel<T, OP, 6><<<grid, 256, 0, s>>>(a); break;
+        default: bmv_rows_kernel<T, OP, 8><<<grid, 256, 0, s>>>(a); break;
+      }
+      BGB_HIP_TRY(hipGetLastError());
+      return BGB_OK;
+    }
+  }
   return g_run_length[ilog2_tiledim(T) - 2] == 16 ? launch_bmv_s<T, OP, 16>(a, s) : launch_bmv_s<T, OP, 8>(a, s);
 }
 
@@ -782,6 +915,59 @@ int device_facts(DeviceFacts *out) {
   return BGB_OK;
 }
 
+// Most tiles in one block row, per (device, rowptr address, nbr, nblocks):
+// one reduction pass and a read-back on the first call for a matrix (so the
+// first call synchronises the stream), kept for the next. Unknown (-1) while
+// the stream is being captured into a graph. A caller that rewrites rowptr in
+// place keeps the old figure: it only selects a kernel, both are exact.
+namespace {
+struct RowStat {
+  int dev;
+  const int *rowptr;
+  int nbr, nblocks, max_row;
+};
+constexpr int kRowStatCache = 16;
+std::mutex g_rowstat_mu;
+std::vector<RowStat> g_rowstat;  // most recently used last
+}  // namespace
+
+int max_row_tiles(hipStream_t s, const int *rowptr, int nbr, int nblocks, int *out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_rowstat_mu);
+    for (size_t i = 0; i < g_rowstat.size(); ++i) {
+      const RowStat e = g_rowstat[i];
+      if (e.dev == dev && e.rowptr == rowptr && e.nbr == nbr && e.nblocks == nblocks) {
+        g_rowstat.erase(g_rowstat.begin() + (long)i);
+        g_rowstat.push_back(e);
+        *out = e.max_row;
+        return BGB_OK;
+      }
+    }
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  BGB_HIP_TRY(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    *out = -1;
+    return BGB_OK;
+  }
+  void *ws = nullptr;
+  int st = stream_workspace(s, 256, &ws);
+  if (st) return st;
+  BGB_HIP_TRY(hipMemsetAsync(ws, 0, sizeof(int), s));
+  max_row_tiles_kernel<<<capped_blocks(nbr, 256), 256, 0, s>>>(rowptr, nbr, static_cast<int *>(ws));
+  BGB_HIP_TRY(hipGetLastError());
+  int h = 0;
+  BGB_HIP_TRY(hipMemcpyAsync(&h, ws, sizeof(int), hipMemcpyDeviceToHost, s));
+  BGB_HIP_TRY(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(g_rowstat_mu);
+  if ((int)g_rowstat.size() >= kRowStatCache) g_rowstat.erase(g_rowstat.begin());
+  g_rowstat.push_back(RowStat{dev, rowptr, nbr, nblocks, h});
+  *out = h;
+  return BGB_OK;
+}
+
 // Chunk partitions (chunk -> block row of its first tile), one per (device,
 // stream, rowptr address, nbr, nblocks, chunk size), kept for the next call on
 // the same matrix. The kernels check every entry against rowptr and repair
@@ -871,6 +1057,15 @@ int bgb_set_x_staging(int T, int mode) {
 }
 
 int bgb_get_x_staging(int T) { return valid_tiledim(T) ? g_x_staging[ilog2_tiledim(T) - 2] : -1; }
+
+int bgb_set_short_row_path(int mode) {
+  clear_error();
+  BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "short-row path mode must be 0, 1 or 2 (got %d)", mode);
+  g_short_rows = mode;
+  return BGB_OK;
+}
+
+int bgb_get_short_row_path(void) { return g_short_rows; }
 
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C, const int *d_rowptr,
                         const int *d_colind, int nblockrows, int nblocks, int accumulate,

@@ -44,6 +44,7 @@ int stream_workspace(hipStream_t s, size_t bytes, void **out);
 // Chunk -> first block row for chunks of CH tiles, cached per (device, stream,
 // rowptr address, sizes); the kernels validate and repair entries.
 int chunk_partition(hipStream_t s, const int *rowptr, int nbr, int nblocks, int CH, int **out);
+int max_row_tiles(hipStream_t s, const int *rowptr, int nbr, int nblocks, int *out);
 
 // Per-device launch facts, queried once per device: compute units and the
 // LDS bytes a workgroup / a CU may use.

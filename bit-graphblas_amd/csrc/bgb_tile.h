@@ -57,6 +57,14 @@ __device__ __forceinline__ int bit_count(uint64_t w) { return __popcll(w); }
 __device__ __forceinline__ int top_bit(uint32_t w) { return 31 - __clz((int)w); }
 __device__ __forceinline__ int top_bit(uint64_t w) { return 63 - __clzll((long long)w); }
 
+// x where bit b of w is set, else +0.0 (whose bits do not depend on x, so a
+// non-finite x in a clear column never reaches a sum): the sign-extended
+// 1-bit field of w is an all-ones or all-zero mask -- an extract and an AND,
+// no compare, no condition-register hazard.
+__device__ __forceinline__ float bit_select(uint32_t w, int b, float x) {
+  return __uint_as_float(__float_as_uint(x) & (uint32_t)__builtin_amdgcn_sbfe((int)w, b, 1));
+}
+
 // 16-byte non-temporal load (streamed data that is read once per pass).
 __device__ __forceinline__ uint4 load_nt16(const void *p) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));

@@ -133,6 +133,19 @@ int bgb_get_run_length(int T);
 int bgb_set_x_staging(int T, int mode);
 int bgb_get_x_staging(int T);
 
+/* Short-row path for T = 4 and 8, every BMV op: one lane per block row walks
+ * that row's tiles and writes it once (no chunk partition, carries or
+ * fix-up). 0 = automatic: taken when no block row holds more than 32 tiles
+ * (regular matrices such as 2-D grids); the figure is measured on the first
+ * call for a (rowptr, nblockrows, nblocks) -- one reduction and a stream
+ * synchronisation -- and kept; while the stream is being captured into a
+ * graph it is not measured and the chunked kernels run. 1 = never, 2 =
+ * always (exact for any matrix, slow for long rows). Process-wide tuning
+ * knob (no reference counterpart; the reference's kernels are one warp per
+ * 1-8 block rows, bsrbmv.cu:298-303). Default 0. */
+int bgb_set_short_row_path(int mode);
+int bgb_get_short_row_path(void);
+
 /* C[I] (|)= bits(T-1-r) = OR_c A[I*T+r][c] & x[c].  bmv{T}_bin_bin_bin
  * (bsrbmv.cu:277,437,508,579). B: packed bit vector, C: W_T[NBR]. */
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C,

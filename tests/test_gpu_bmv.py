@@ -246,6 +246,59 @@ def test_rowptr_rewritten_in_place(orc, dev, T):
         np.testing.assert_array_equal(to_np(B.bmv_bin_bin_full(A1, xb))[:n], want2)
 
 
+@pytest.mark.parametrize("T", (4, 8))
+def test_short_row_path_modes(orc, dev, T):
+    """T <= 8 BMVs run the chunked kernels or one lane per block row (taken
+    automatically when no block row holds more than 32 tiles). Both must give
+    the oracle's results on short rows (a grid), on rows far longer than one
+    batch of tiles (R-MAT hubs, forced onto the short-row path), on a ragged n
+    and on runs of empty block rows; accumulate=1 adds onto the output."""
+    import torch
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(90 + T)
+    src = np.concatenate([rng.integers(0, 100, 400), rng.integers(200, 300, 400)])
+    dst = np.concatenate([rng.integers(0, 100, 400), rng.integers(200, 300, 400)])
+    cases = (("grid 70x33", graph.grid2d(70, 33)), ("rmat14", graph.rmat(14, 8, seed=12)),
+             ("ragged n=37", graph.erdos_renyi(37, 6, seed=4)),
+             ("empty block rows", graph.from_coo(300, 300, src, dst)))
+    try:
+        for mode in (1, 2):
+            bgb.call("bgb_set_short_row_path", mode)
+            for label, csr in cases:
+                A, xb, xfp = check_all_ops(orc, csr, T, rng, dev, f"short rows mode {mode} {label}")
+                n = csr.nrows
+                once = B.bmv_bin_bin_full(A, xb)
+                twice = B.bmv_bin_bin_full(A, xb, out=once.clone(), accumulate=True)
+                np.testing.assert_array_equal(to_np(twice), 2 * to_np(once))
+                bits = B.bmv_bin_bin_bin(A, xb)
+                np.testing.assert_array_equal(to_np(B.bmv_bin_bin_bin(A, xb, out=bits.clone(), accumulate=True), T),
+                                              to_np(bits, T))
+                f1 = B.bmv_bin_full_full(A, xfp)
+                f2 = B.bmv_bin_full_full(A, xfp, out=f1.clone(), accumulate=True)
+                np.testing.assert_allclose(to_np(f2), 2 * to_np(f1), rtol=1e-6, atol=1e-6)
+                xh = xfp.to(torch.bfloat16)
+                want = orc.csr_f64(csr.rowptr, csr.colind, xh.float().cpu().numpy()[:n].astype(np.float64))
+                got = to_np(B.bmv_bin_full_full_bf16(A, xh))[:n]
+                np.testing.assert_allclose(got, want, rtol=FP_RTOL, atol=FP_ATOL)
+    finally:
+        bgb.call("bgb_set_short_row_path", 0)
+    # automatic mode: the batch of tiles per load round follows the longest
+    # block row (4, 5, 6 or 8 tiles, several batches past 8, the chunked
+    # kernels past 32): block rows of exactly k tiles, k columns drawn per row
+    nbr = 300
+    for k in (1, 3, 5, 6, 7, 9, 17, 32, 33):
+        rows, cols = [], []
+        for I in range(nbr):
+            J = rng.choice(nbr, k, replace=False)
+            for j in J:
+                rows.append(I * T + rng.integers(0, T, 2))
+                cols.append(j * T + rng.integers(0, T, 2))
+        csr = graph.from_coo(nbr * T, nbr * T, np.concatenate(rows), np.concatenate(cols), False, False)
+        check_all_ops(orc, csr, T, rng, dev, f"auto path, {k} tiles per block row")
+
+
 @pytest.mark.parametrize("T", TS)
 def test_x_staging_modes(orc, dev, T):
     """The bit BMVs read the packed x either from memory or from a copy staged
