@@ -45,8 +45,6 @@ enum BmvOp : int {
   OP_FULL_BF16 = 5  // x in bfloat16, FP32 accumulation
 };
 
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
 struct BmvArgs {
   const void *A;
   const void *B;
@@ -627,8 +625,6 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
 // window, segmented scan or carries. For regular matrices (a 2-D grid has 2-3
 // tiles per block row) the chunk walk would otherwise spend most of its time
 // on row ends: a run of 8 tiles crosses ~3 of them.
-constexpr int kShortRowMax = 32;
-
 // U: tiles per batch of loads, picked from the longest block row (a row of
 // at most 8 tiles is one batch with the fewest padding slots: on the 4096^2
 // grid, 5 tiles per block row, batches of 5 run T = 8 in 67 us, of 6 in 72,
@@ -636,80 +632,14 @@ constexpr int kShortRowMax = 32;
 template <int T, int OP, int U>
 __global__ __launch_bounds__(256) void bmv_rows_kernel(BmvArgs a) {
   static_assert(Tile<T>::L == 1, "one lane per block row needs T <= 8");
-  constexpr int R = Tile<T>::R, NW = Tile<T>::NW;
+  constexpr int kMode = OP == OP_FULL_F32 ? kRowSumF32 : OP == OP_FULL_BF16 ? kRowSumBF16 : kRowCount;
   using Acc = typename AccOf<OP>::type;
-  using W = typename Tile<T>::W;
   // consecutive block rows on one XCD: the x segments a row gathers (for a
   // grid, block columns I and I +- one grid row) stay in that XCD's L2
   for (long long I = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; I < a.nbr;
        I += (long long)gridDim.x * blockDim.x) {
-    const int s = a.rowptr[I], e = a.rowptr[I + 1];
-    Acc acc[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) acc[i] = 0;
-    for (int k = s; k < e; k += U) {
-      uint32_t F[U][NW];
-      int J[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const long long p = k + u < e ? k + u : e - 1;
-        J[u] = a.colind[p];
-        load_frag<T>(a.A, p, 0, F[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (k + u >= e) {
-#pragma unroll
-          for (int w = 0; w < NW; ++w) F[u][w] = 0u;  // padding: no set bits
-        }
-      if constexpr (is_full(OP)) {
-        float XS[U][T];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if constexpr (OP == OP_FULL_F32) {
-            const float4 *xp = reinterpret_cast<const float4 *>(static_cast<const float *>(a.B) + (long long)J[u] * T);
-#pragma unroll
-            for (int v = 0; v < T / 4; ++v) {
-              const float4 f = xp[v];
-              XS[u][4 * v] = f.x;
-              XS[u][4 * v + 1] = f.y;
-              XS[u][4 * v + 2] = f.z;
-              XS[u][4 * v + 3] = f.w;
-            }
-          } else {
-            const uint2 *xp = reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.B) + (long long)J[u] * T);
-#pragma unroll
-            for (int v = 0; v < T / 4; ++v) {
-              const uint2 h = xp[v];
-              XS[u][4 * v] = bf16_to_f32((uint16_t)(h.x & 0xFFFFu));
-              XS[u][4 * v + 1] = bf16_to_f32((uint16_t)(h.x >> 16));
-              XS[u][4 * v + 2] = bf16_to_f32((uint16_t)(h.y & 0xFFFFu));
-              XS[u][4 * v + 3] = bf16_to_f32((uint16_t)(h.y >> 16));
-            }
-          }
-        }
-        // tiles in order, ascending columns within a tile, set bits only:
-        // the CSR summation order. (Skipping slots that are padding in every
-        // lane behind a wave-uniform branch let the compiler sink each slot's
-        // loads into its branch, one round trip per tile: slower.)
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int i = 0; i < R; ++i) {
-            const Word<T> w = row_word<T>(F[u], i);
-#pragma unroll
-            for (int c = 0; c < T; ++c) acc[i] += bit_select(w, T - 1 - c, XS[u][c]);
-          }
-      } else {
-        Word<T> X[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) X[u] = (Word<T>) static_cast<const W *>(a.B)[J[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int i = 0; i < R; ++i) acc[i] += bit_count(row_word<T>(F[u], i) & X[u]);
-      }
-    }
+    Acc acc[Tile<T>::R];
+    block_row_sums<T, kMode, U>(a.A, a.colind, a.B, a.rowptr[I], a.rowptr[I + 1], acc);
     flush_row<T, OP>(a, (int)I, 0, acc);
   }
 }
@@ -807,7 +737,7 @@ int launch_bmv(BmvArgs a, hipStream_t s) {
     }
     if (mr >= 0 && mr <= kShortRowMax) {
       const unsigned grid = capped_blocks(a.nbr, 256);
-      switch (mr <= 4 ? 4 : mr <= 6 ? mr : 8) {
+      switch (short_row_batch(mr)) {
         case 4: bmv_rows_kernel<T, OP, 4><<<grid, 256, 0, s>>>(a); break;
         case 5: bmv_rows_kernel<T, OP, 5><<<grid, 256, 0, s>>>(a); break;
         case 6: bmv_rows_kernel<T, OP, 6><<<grid, 256, 0, s>>>(a); break;

@@ -65,6 +65,8 @@ __device__ __forceinline__ float bit_select(uint32_t w, int b, float x) {
   return __uint_as_float(__float_as_uint(x) & (uint32_t)__builtin_amdgcn_sbfe((int)w, b, 1));
 }
 
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 // 16-byte non-temporal load (streamed data that is read once per pass).
 __device__ __forceinline__ uint4 load_nt16(const void *p) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -640,6 +642,91 @@ __device__ __forceinline__ void group_scan(int key, Acc (&sc)[R], Op op) {
 __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// One block row's results for the short-row path (T <= 8, one lane per block
+// row): the row's tiles [s, e) in batches of U, every load of a batch issued
+// unconditionally (indices clamped into the row, the padding slots' words
+// zeroed so they add nothing). kMode: popcount(a_r & x_J) counts over packed
+// x (x: W_T words), or FP sums over an FP32 / bfloat16 x (T values per block
+// column) in the CSR order -- tiles in order, ascending columns, set bits only.
+constexpr int kRowCount = 0, kRowSumF32 = 1, kRowSumBF16 = 2;
+// The short-row path applies when no block row holds more than this many
+// tiles; its batch U follows the longest row (4 up to 4 tiles, then 5, 6, 8).
+constexpr int kShortRowMax = 32;
+inline int short_row_batch(int max_row) { return max_row <= 4 ? 4 : max_row <= 6 ? max_row : 8; }
+
+template <int T, int kMode, int U, class Acc>
+__device__ __forceinline__ void block_row_sums(const void *__restrict__ A, const int *__restrict__ colind,
+                                               const void *__restrict__ x, int s, int e,
+                                               Acc (&acc)[Tile<T>::R]) {
+  static_assert(Tile<T>::L == 1, "one lane per block row needs T <= 8");
+  constexpr int R = Tile<T>::R, NW = Tile<T>::NW;
+  using W = typename Tile<T>::W;
+#pragma unroll
+  for (int i = 0; i < R; ++i) acc[i] = 0;
+  for (int k = s; k < e; k += U) {
+    uint32_t F[U][NW];
+    int J[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long p = k + u < e ? k + u : e - 1;
+      J[u] = colind[p];
+      load_frag<T>(A, p, 0, F[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k + u >= e) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) F[u][w] = 0u;  // padding: no set bits
+      }
+    if constexpr (kMode != kRowCount) {
+      float XS[U][T];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if constexpr (kMode == kRowSumF32) {
+          const float4 *xp = reinterpret_cast<const float4 *>(static_cast<const float *>(x) + (long long)J[u] * T);
+#pragma unroll
+          for (int v = 0; v < T / 4; ++v) {
+            const float4 f = xp[v];
+            XS[u][4 * v] = f.x;
+            XS[u][4 * v + 1] = f.y;
+            XS[u][4 * v + 2] = f.z;
+            XS[u][4 * v + 3] = f.w;
+          }
+        } else {
+          const uint2 *xp = reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(x) + (long long)J[u] * T);
+#pragma unroll
+          for (int v = 0; v < T / 4; ++v) {
+            const uint2 h = xp[v];
+            XS[u][4 * v] = bf16_to_f32((uint16_t)(h.x & 0xFFFFu));
+            XS[u][4 * v + 1] = bf16_to_f32((uint16_t)(h.x >> 16));
+            XS[u][4 * v + 2] = bf16_to_f32((uint16_t)(h.y & 0xFFFFu));
+            XS[u][4 * v + 3] = bf16_to_f32((uint16_t)(h.y >> 16));
+          }
+        }
+      }
+      // (Skipping slots that are padding in every lane behind a wave-uniform
+      // branch let the compiler sink each slot's loads into its branch, one
+      // round trip per tile: slower.)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const Word<T> w = row_word<T>(F[u], i);
+#pragma unroll
+          for (int c = 0; c < T; ++c) acc[i] += bit_select(w, T - 1 - c, XS[u][c]);
+        }
+    } else {
+      Word<T> X[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) X[u] = (Word<T>) static_cast<const W *>(x)[J[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i] += bit_count(row_word<T>(F[u], i) & X[u]);
+    }
+  }
 }
 
 }  // namespace bgb

@@ -144,6 +144,35 @@ def test_pagerank_grid_stride(orc, dev):
         np.testing.assert_allclose(res2[:8], res[:8], rtol=1e-3)
 
 
+def test_pagerank_fused_short_rows(orc, dev):
+    """A grid has 5 tiles per block row at T = 4 and 8, so each iteration is one
+    fused kernel (short-row BMV + update, x double-buffered); 2048 x 2100
+    vertices give more block rows than one pass of its capped grid covers
+    (4096 workgroups x 256) at T = 4. Checked against the oracle and against
+    the unfused iteration (chunked BMV, then the update kernel)."""
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    csr = graph.grid2d(2048, 2100)
+    assert -(-csr.nrows // 4) > 4096 * 256
+    ref, rres = orc.pagerank(csr.rowptr, csr.colind, 0.85, 20)
+    for T in (4, 8):
+        A = B.B2SR.from_csr(csr, T, dev)
+        for mode in ("fp32", "bf16", "fp64sum"):
+            log(f"[case] pagerank fused grid T={T} mode={mode}")
+            pr, res = B.pagerank(A, 0.85, 20, mode)
+            assert pr_errors(pr.cpu().numpy(), ref) < PR_RTOL[mode]
+            if mode != "bf16":
+                np.testing.assert_allclose(res[:8], rres[:8], rtol=1e-3)
+            try:
+                bgb.call("bgb_set_short_row_path", 1)
+                pr_u, _ = B.pagerank(A, 0.85, 20, mode)
+            finally:
+                bgb.call("bgb_set_short_row_path", 0)
+            np.testing.assert_allclose(pr.cpu().numpy(), pr_u.cpu().numpy(),
+                                       rtol=1e-5 if mode != "bf16" else 2e-2)
+
+
 def test_pagerank_rmat_with_dangling(orc, dev):
     from bgb import b2sr as B
     from bgb import graph
