@@ -238,6 +238,64 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   // common case) or from rowptr in memory (chunks spanning > kWin rows).
   rw.dispatch([&](auto st) {
   constexpr bool ST = decltype(st)::value;
+  // A chunk inside one block row (not the first or last chunk, and the next
+  // chunk starts in the same row, so the row neither starts nor ends here --
+  // 46 % of the chunks at T = 4 on R-MAT 20, inside hub rows): no row ends to
+  // find or flush, no segmented scan; the counts are summed over the wave and
+  // published as the chunk's carry record. Count and bit ops (integer sums,
+  // exact in any order).
+  if constexpr (ST && !is_full(OP)) {
+    if (k0 > 0 && chunk + 1 < a.nchunks && rw.Ia == rw.Ib) {  // wave-uniform
+      Acc tot[R];
+      if constexpr (T == 4) {
+        uint32_t pk = 0;  // four byte-wise row counts, at most 8 x 4 each
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          uint32_t m = F[s][0] & ((uint32_t)X[s] * 0x01010101u);
+          m = m - ((m >> 1) & 0x55555555u);
+          m = (m & 0x33333333u) + ((m >> 2) & 0x33333333u);
+          pk += m;
+        }
+        // rows 0, 2 and 1, 3 in 16-bit fields (the wave sum is at most 2048)
+        uint32_t lo = pk & 0x00FF00FFu, hi = (pk >> 8) & 0x00FF00FFu;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+          lo += (uint32_t)__shfl_xor((int)lo, m);
+          hi += (uint32_t)__shfl_xor((int)hi, m);
+        }
+        tot[0] = lo & 0xFFFFu, tot[1] = hi & 0xFFFFu, tot[2] = lo >> 16, tot[3] = hi >> 16;
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) tot[i] = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+          for (int i = 0; i < R; ++i) tot[i] += bit_count(row_word<T>(F[s], i) & X[s]);
+#pragma unroll
+        for (int m = L; m < 64; m <<= 1)
+#pragma unroll
+          for (int i = 0; i < R; ++i) tot[i] += (Acc)__shfl_xor((int)tot[i], m);
+      }
+      if (g == G - 1) {
+        bool open = true;
+        if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+          bool nz = false;
+#pragma unroll
+          for (int i = 0; i < R; ++i) nz |= tot[i] != (Acc)0;
+#pragma unroll
+          for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
+          open = nz;
+        }
+        if (q == 0) a.carry_key[chunk] = open ? rw.Ia : -1;
+        if (open) {
+          Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
+#pragma unroll
+          for (int i = 0; i < R; ++i) cv[i] = tot[i];
+        }
+      }
+      return;
+    }
+  }
   int I;
   if constexpr (ST) {
     // the registers hold the window unless put_checked restaged it
