@@ -304,16 +304,21 @@ def test_x_staging_modes(orc, dev, T):
     """The bit BMVs read the packed x either from memory or from a copy staged
     in LDS by persistent workgroups (T=4 packs two nibbles per LDS byte): both
     paths must give the oracle's results, including packed-x lengths that are
-    not a multiple of 16 bytes (the staging copies a tail)."""
+    not a multiple of 16 bytes (the staging copies a tail), and chunks that
+    lie inside one block row (a star's hub row spans many chunks; such chunks
+    publish a wave sum as their carry)."""
     import bgb
     from bgb import graph
     rng = np.random.default_rng(70 + T)
+    n = 40000
+    star = graph.from_coo(n, n, np.zeros(n - 1, np.int32), np.arange(1, n, dtype=np.int32), True, True)
     try:
         for mode in (1, 2):
             bgb.call("bgb_set_x_staging", T, mode)
             for label, csr in (("rmat14", graph.rmat(14, 8, seed=12)),
                                ("ragged n=1001", graph.erdos_renyi(1001, 12, seed=3)),
-                               ("ragged n=37", graph.erdos_renyi(37, 6, seed=4))):
+                               ("ragged n=37", graph.erdos_renyi(37, 6, seed=4)),
+                               ("star n=40000", star)):
                 check_all_ops(orc, csr, T, rng, dev, f"x staging mode {mode} {label}")
     finally:
         bgb.call("bgb_set_x_staging", T, 0)
