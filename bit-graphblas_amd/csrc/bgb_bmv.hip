@@ -238,59 +238,85 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   // common case) or from rowptr in memory (chunks spanning > kWin rows).
   rw.dispatch([&](auto st) {
   constexpr bool ST = decltype(st)::value;
-  // A chunk inside one block row (not the first or last chunk, and the next
-  // chunk starts in the same row, so the row neither starts nor ends here --
-  // 46 % of the chunks at T = 4 on R-MAT 20, inside hub rows): no row ends to
-  // find or flush, no segmented scan; the counts are summed over the wave and
-  // published as the chunk's carry record. Count and bit ops (integer sums,
-  // exact in any order).
+  // A chunk over one or two block rows (not the first or last chunk; 46 % and
+  // 19 % of the chunks at T = 4 on R-MAT 20): rows Ia = Ib (the row neither
+  // starts nor ends here, the next chunk starts in it) or Ib = Ia + 1 (row Ia
+  // ends at E inside the chunk, row Ib runs on into the next one). Each lane
+  // splits its run's counts at E, two wave sums give the rows' totals, row Ia
+  // is flushed and row Ib (Ia for one row) becomes the chunk's carry record
+  // -- no group-row pass, row-end walk or segmented scan. Count and bit ops
+  // (integer sums, exact in any order).
+  // Two rows only where x is gathered from memory: the LDS-staged kernels run
+  // 16 waves of at most 128 VGPRs, and the second row's sums made T = 8 spill
+  // there (R-MAT 20 T = 8: 99 -> 113 us; from memory, R-MAT 22 T = 8 510 -> 482).
+  constexpr int kMaxRows = XL ? 1 : 2;
   if constexpr (ST && !is_full(OP)) {
-    if (k0 > 0 && chunk + 1 < a.nchunks && rw.Ia == rw.Ib) {  // wave-uniform
-      Acc tot[R];
+    if (k0 > 0 && chunk + 1 < a.nchunks && rw.Ib - rw.Ia < kMaxRows) {  // wave-uniform
+      const bool two = kMaxRows == 2 && rw.Ib != rw.Ia;
+      const long long E = two ? (long long)rw.at_t<true>(rw.Ia + 1) : LLONG_MAX;
+      Acc ta[R], tb[R];  // row Ia (tiles before E), row Ib (tiles from E)
       if constexpr (T == 4) {
-        uint32_t pk = 0;  // four byte-wise row counts, at most 8 x 4 each
+        uint32_t pka = 0, pkb = 0;  // four byte-wise row counts, at most 8 x 4 each
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           uint32_t m = F[s][0] & ((uint32_t)X[s] * 0x01010101u);
           m = m - ((m >> 1) & 0x55555555u);
           m = (m & 0x33333333u) + ((m >> 2) & 0x33333333u);
-          pk += m;
+          const bool before = pa + s < E;
+          pka += before ? m : 0u;
+          pkb += before ? 0u : m;
         }
-        // rows 0, 2 and 1, 3 in 16-bit fields (the wave sum is at most 2048)
-        uint32_t lo = pk & 0x00FF00FFu, hi = (pk >> 8) & 0x00FF00FFu;
+        // rows 0, 2 and 1, 3 in 16-bit fields (a wave sum is at most 2048)
+        auto wave_rows = [&](uint32_t pk, Acc (&t)[R]) {
+          uint32_t lo = pk & 0x00FF00FFu, hi = (pk >> 8) & 0x00FF00FFu;
 #pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-          lo += (uint32_t)__shfl_xor((int)lo, m);
-          hi += (uint32_t)__shfl_xor((int)hi, m);
-        }
-        tot[0] = lo & 0xFFFFu, tot[1] = hi & 0xFFFFu, tot[2] = lo >> 16, tot[3] = hi >> 16;
+          for (int m = 1; m < 64; m <<= 1) {
+            lo += (uint32_t)__shfl_xor((int)lo, m);
+            hi += (uint32_t)__shfl_xor((int)hi, m);
+          }
+          t[0] = lo & 0xFFFFu, t[1] = hi & 0xFFFFu, t[2] = lo >> 16, t[3] = hi >> 16;
+        };
+        wave_rows(pka, ta);
+        if (two) wave_rows(pkb, tb);
       } else {
 #pragma unroll
-        for (int i = 0; i < R; ++i) tot[i] = 0;
+        for (int i = 0; i < R; ++i) ta[i] = tb[i] = 0;
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
+          const bool before = pa + s < E;
 #pragma unroll
-          for (int i = 0; i < R; ++i) tot[i] += bit_count(row_word<T>(F[s], i) & X[s]);
+          for (int i = 0; i < R; ++i) {
+            const Acc c = (Acc)bit_count(row_word<T>(F[s], i) & X[s]);
+            ta[i] += before ? c : (Acc)0;
+            tb[i] += before ? (Acc)0 : c;
+          }
+        }
+        auto wave_rows = [&](Acc (&t)[R]) {
 #pragma unroll
-        for (int m = L; m < 64; m <<= 1)
+          for (int m = L; m < 64; m <<= 1)
 #pragma unroll
-          for (int i = 0; i < R; ++i) tot[i] += (Acc)__shfl_xor((int)tot[i], m);
+            for (int i = 0; i < R; ++i) t[i] += (Acc)__shfl_xor((int)t[i], m);
+        };
+        wave_rows(ta);
+        if (two) wave_rows(tb);
       }
+      if (two && g == 0) flush_row<T, OP>(a, rw.Ia, q, ta);
+      Acc *open_t = two ? tb : ta;
       if (g == G - 1) {
         bool open = true;
         if constexpr (OP == OP_BITS || OP == OP_MASKED) {
           bool nz = false;
 #pragma unroll
-          for (int i = 0; i < R; ++i) nz |= tot[i] != (Acc)0;
+          for (int i = 0; i < R; ++i) nz |= open_t[i] != (Acc)0;
 #pragma unroll
           for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
           open = nz;
         }
-        if (q == 0) a.carry_key[chunk] = open ? rw.Ia : -1;
+        if (q == 0) a.carry_key[chunk] = open ? rw.Ib : -1;
         if (open) {
           Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
 #pragma unroll
-          for (int i = 0; i < R; ++i) cv[i] = tot[i];
+          for (int i = 0; i < R; ++i) cv[i] = open_t[i];
         }
       }
       return;
