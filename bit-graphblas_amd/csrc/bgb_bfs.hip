@@ -257,34 +257,44 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   });
 }
 
-// Folds the carries of rows that cross chunks into this level's results.
+// Folds the carries of rows that cross chunks into this level's results. One
+// wave per 64 consecutive chunks, lane j owning chunk 64w + j: carry keys are
+// non-decreasing over chunks (none only past the last block row), so the
+// chunks carrying into one row are contiguous; a segmented OR-scan over the
+// lanes joins each run and only the run's last lane records the new bits
+// (atomics, since a run can continue into a neighbouring wave). One thread
+// per record instead sent a hub row's hundreds of records to the same
+// addresses: 190-240 us on the busiest levels of R-MAT 22.
 template <int T>
-__global__ void bfs_fixup_kernel(BfsState b) {
+__global__ __launch_bounds__(256) void bfs_fixup_kernel(BfsState b) {
   using W = typename Tile<T>::W;
   if (b.ctl[2] || b.ctl[3]) return;
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long long)b.nchunks * T) return;
-  const long long c = tid / T;
-  const int r = (int)(tid % T);
-  const int key = b.carry_key[c];
-  if (key < 0) return;
-  const long long v = (long long)key * T + r;
-  const uint64_t cv = b.carry_val[tid];
-  if (!cv) return;
-  const uint64_t vold = b.V[v];
-  const uint64_t nb = cv & ~vold;
-  if (!nb) return;
+  const int lane = threadIdx.x & 63;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int key = c < b.nchunks ? b.carry_key[c] : -1;
+  key = key < 0 ? INT_MAX : key;  // no record: past every real key
+  const int next = __shfl(key, lane + 1 < 64 ? lane + 1 : 63);
+  const bool last_of_run = lane == 63 || next != key;
   const int depth = b.ctl[0];
-  const unsigned long long fold = atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
-  if (!fold && b.rowptrT)
-    atomicAdd(b.fcnt + (tid & (kCounters - 1)), (unsigned long long)(b.rowptrT[key + 1] - b.rowptrT[key]));
-  atomicOr(reinterpret_cast<unsigned long long *>(b.V + v), (unsigned long long)nb);
-  for (int k = 0; k < kPlanes; ++k)
-    if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), (unsigned long long)nb);
-  const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(b.fnzn) + key);
-  atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3),
-           (1u << (T - 1 - r)) << ((int)(addr & 3) * 8));
-  b.ctl[1] = 1;
+  bool any = false;
+  for (int r = 0; r < T; ++r) {
+    uint64_t sc[1] = {key != INT_MAX ? b.carry_val[c * T + r] : 0ull};
+    group_scan<1, 1, uint64_t>(key, sc, [](uint64_t x, uint64_t y) { return x | y; });
+    if (!last_of_run || key == INT_MAX || !sc[0]) continue;
+    const long long v = (long long)key * T + r;
+    const uint64_t nb = sc[0] & ~b.V[v];
+    if (!nb) continue;
+    const unsigned long long fold = atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
+    if (!fold && b.rowptrT)
+      atomicAdd(b.fcnt + (c & (kCounters - 1)), (unsigned long long)(b.rowptrT[key + 1] - b.rowptrT[key]));
+    atomicOr(reinterpret_cast<unsigned long long *>(b.V + v), (unsigned long long)nb);
+    for (int k = 0; k < kPlanes; ++k)
+      if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), (unsigned long long)nb);
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(static_cast<W *>(b.fnzn) + key);
+    atomicOr(reinterpret_cast<unsigned int *>(addr & ~(uintptr_t)3), (1u << (T - 1 - r)) << ((int)(addr & 3) * 8));
+    any = true;
+  }
+  if (any) b.ctl[1] = 1;
 }
 
 // End of a level: stop when nothing new was found, else advance the depth and
@@ -578,7 +588,7 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     b.big_list = bigl.as<int2>();
   }
   const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
-  const unsigned fblocks = nblk(nchunks * T);
+  const unsigned fblocks = nblk(ceil_div(nchunks, 64) * 64);  // one lane per chunk, whole waves
   const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 8192);
   const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
   const long long zbytes = wb * nwords + 16;
