@@ -85,11 +85,10 @@ def test_k16_persistent_batch(orc, dev):
 
 
 @pytest.mark.timeout(600)
-def test_k16_batch_fixup_grid_stride(orc, dev):
-    """A batch whose carry fix-up outgrows its capped grid (three R-MAT scale-22
-    graphs at T=16, 256-tile chunks, 512 carry values each): the fix-up grid is
-    capped at 2^20 workgroups of 256 (a dispatch's work-item count is 32 bits),
-    so every thread walks the range in more than one stride."""
+def test_k16_batch_rmat22(orc, dev):
+    """Three R-MAT scale-22 graphs at T=16 in one persistent launch (about a
+    million 256-tile chunks; hub rows carry across many chunks and across the
+    fix-up's 64-chunk waves): every graph within the FP32 bound."""
     from bgb import b2sr as B
     from bgb import graph
     mats, xbs, refs = [], [], []
@@ -102,7 +101,6 @@ def test_k16_batch_fixup_grid_stride(orc, dev):
         xbs.append(B.k16_pack_rhs(X, A.nblockrows * 16))
         refs.append((csr, X64))
         chunks += -(-A.nblocks // 256)
-    assert chunks * 512 > (1 << 20) * 256
     log(f"[case] k16 batch of 3 R-MAT 22 graphs: {chunks} chunks")
     batch = B.K16Batch(mats, xbs)
     batch.run()
