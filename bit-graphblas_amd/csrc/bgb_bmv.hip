@@ -652,26 +652,34 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
     if (c0 >= a.nchunks) return;  // uniform over the wave
     const long long c = c0 + lane;
     const bool in = c < a.nchunks;
-    int key = in ? a.carry_key[c] : -1;
-    if (key < 0) key = INT_MAX;  // no open row: never written
+    // every load is issued up front, unconditionally (indices clamped, values
+    // of chunks without a record zeroed after), so they share one round trip:
+    // the keys, the records and the neighbouring waves' boundary keys
+    const long long cc = in ? c : a.nchunks - 1;
+    int key = a.carry_key[cc];
+    const int kprev = a.carry_key[c0 > 0 ? c0 - 1 : 0];
+    const int knext_wave = a.carry_key[c0 + 64 < a.nchunks ? c0 + 64 : a.nchunks - 1];
     Acc v[T];
-    const Acc4 *cv = reinterpret_cast<const Acc4 *>(static_cast<const Acc *>(a.carry_val) + c * T);
+    const Acc4 *cv = reinterpret_cast<const Acc4 *>(static_cast<const Acc *>(a.carry_val) + cc * T);
 #pragma unroll
     for (int r = 0; r < T / 4; ++r) {
-      const Acc4 f = key != INT_MAX ? cv[r] : Acc4{0, 0, 0, 0};
+      const Acc4 f = cv[r];
       v[4 * r] = f.x;
       v[4 * r + 1] = f.y;
       v[4 * r + 2] = f.z;
       v[4 * r + 3] = f.w;
     }
+    if (!in || key < 0) key = INT_MAX;  // no open row: never written
+#pragma unroll
+    for (int r = 0; r < T; ++r) v[r] = key != INT_MAX ? v[r] : (Acc)0;
     group_scan<1, T, Acc>(key, v, [](Acc x, Acc y) { return x + y; });
     const int key0 = __shfl(key, 0), kend = __shfl(key, 63);
     const int knext = __shfl_down(key, 1);
     const bool run_end = lane == 63 || knext != key;
     if (!run_end || key == INT_MAX) return;
     // does the run continue from the previous / into the next wave's chunks?
-    const bool from_prev = key == key0 && c0 > 0 && a.carry_key[c0 - 1] == key;
-    const bool into_next = lane == 63 && c0 + 64 < a.nchunks && a.carry_key[c0 + 64] == kend;
+    const bool from_prev = key == key0 && c0 > 0 && kprev == key;
+    const bool into_next = lane == 63 && c0 + 64 < a.nchunks && knext_wave == kend;
     // output element type: int32 counts, else float (counts converted)
     using Out = typename std::conditional<OP == OP_COUNT_I32, int, float>::type;
     typedef Out Out4 __attribute__((ext_vector_type(4)));
