@@ -15,7 +15,8 @@
 // T = 16 is one MFMA per tile; T = 32 is four (two row halves x two column
 // halves). Work decomposition follows the BMV kernels: fixed chunks of tiles
 // per wave, rows flushed when they end, a carry record for the row open at a
-// chunk's end, and a fix-up pass that adds the records with FP32 atomics.
+// chunk's end, and a fix-up pass that adds the records (summed per row run;
+// atomics only where a run crosses the fix-up's wave boundaries).
 // The batched entry point runs many graphs in one persistent launch whose
 // waves take chunks from a global queue until it is drained.
 #include <hip/hip_runtime.h>
