@@ -170,8 +170,11 @@ int bgb_bmv_bin_bin_full_f32(int T, const void *d_A, const void *d_B, float *d_C
  * FP32; B: float[NBR*T]
  * (padded with zeros beyond nrows).  bmv{T}_bin_full_full
  * (bsrbmv.cu:826,899,975,1040). Each row is summed in ascending column order
- * inside a 64-lane chunk; rows that cross chunks are combined with FP32
- * atomics (the reference also uses atomicAdd, bsrbmv.cu:895,971,1036). */
+ * inside a 64-lane chunk; rows that cross chunks are combined by a fix-up
+ * pass, in a fixed order within each 64 chunks and with FP32 atomics across
+ * them (the reference uses atomicAdd, bsrbmv.cu:895,971,1036). On the
+ * short-row path (bgb_set_short_row_path) every row is summed by one lane in
+ * the CSR order. */
 int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C,
                           const int *d_rowptr, const int *d_colind, int nblockrows,
                           int nblocks, int accumulate, bgb_stream_t stream);
