@@ -663,8 +663,13 @@ __device__ __forceinline__ void block_row_sums(const void *__restrict__ A, const
   static_assert(Tile<T>::L == 1, "one lane per block row needs T <= 8");
   constexpr int R = Tile<T>::R, NW = Tile<T>::NW;
   using W = typename Tile<T>::W;
+  // FP sums are folded in three levels -- a batch of U tiles, up to 16
+  // batches, the row -- so that a long row (the path forced on hub rows) is
+  // not one FP32 chain of thousands of terms; the terms keep the CSR order.
+  Acc mid[R];
+  int nbatch = 0;
 #pragma unroll
-  for (int i = 0; i < R; ++i) acc[i] = 0;
+  for (int i = 0; i < R; ++i) acc[i] = mid[i] = 0;
   for (int k = s; k < e; k += U) {
     uint32_t F[U][NW];
     int J[U];
@@ -709,14 +714,27 @@ __device__ __forceinline__ void block_row_sums(const void *__restrict__ A, const
       // (Skipping slots that are padding in every lane behind a wave-uniform
       // branch let the compiler sink each slot's loads into its branch, one
       // round trip per tile: slower.)
+      Acc bs[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) bs[i] = 0;
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           const Word<T> w = row_word<T>(F[u], i);
 #pragma unroll
-          for (int c = 0; c < T; ++c) acc[i] += bit_select(w, T - 1 - c, XS[u][c]);
+          for (int c = 0; c < T; ++c) bs[i] += bit_select(w, T - 1 - c, XS[u][c]);
         }
+#pragma unroll
+      for (int i = 0; i < R; ++i) mid[i] += bs[i];
+      if (++nbatch == 16) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          acc[i] += mid[i];
+          mid[i] = 0;
+        }
+        nbatch = 0;
+      }
     } else {
       Word<T> X[U];
 #pragma unroll
@@ -726,6 +744,10 @@ __device__ __forceinline__ void block_row_sums(const void *__restrict__ A, const
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] += bit_count(row_word<T>(F[u], i) & X[u]);
     }
+  }
+  if constexpr (kMode != kRowCount) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i] += mid[i];
   }
 }
 

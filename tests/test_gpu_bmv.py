@@ -260,9 +260,13 @@ def test_short_row_path_modes(orc, dev, T):
     rng = np.random.default_rng(90 + T)
     src = np.concatenate([rng.integers(0, 100, 400), rng.integers(200, 300, 400)])
     dst = np.concatenate([rng.integers(0, 100, 400), rng.integers(200, 300, 400)])
+    star = graph.from_coo(30000, 30000, np.zeros(29999, np.int32), np.arange(1, 30000, dtype=np.int32))
     cases = (("grid 70x33", graph.grid2d(70, 33)), ("rmat14", graph.rmat(14, 8, seed=12)),
              ("ragged n=37", graph.erdos_renyi(37, 6, seed=4)),
-             ("empty block rows", graph.from_coo(300, 300, src, dst)))
+             ("empty block rows", graph.from_coo(300, 300, src, dst)),
+             # a 30000-entry row walked by one lane when forced: its FP32 sum is
+             # folded per batch / 16 batches / row, not one chain of 30000 adds
+             ("star n=30000", star))
     try:
         for mode in (1, 2):
             bgb.call("bgb_set_short_row_path", mode)
