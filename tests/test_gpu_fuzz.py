@@ -1,6 +1,7 @@
-"""Seeded randomized parity of every BMV operator against the oracle, across
-the kernel paths a matrix can take: chunked walk (general, one-row and
-two-row chunks), short-row kernel, packed x in LDS or gathered from memory.
+"""Seeded randomized parity against the oracle: every BMV operator across the
+kernel paths a matrix can take (chunked walk with general, one-row and
+two-row chunks; short-row kernel; packed x in LDS or gathered from memory),
+the batched BFS, and the K=16 MFMA path single and batched.
 
 Each case draws a graph shape (hub rows spanning many chunks, runs of empty
 block rows, ragged n, non-square, near-regular rows), a tile dim and the
@@ -70,3 +71,64 @@ def test_random_paths(orc, cuda, seed):
         for T in TS:
             bgb.call("bgb_set_x_staging", T, 0)
         bgb.call("bgb_set_short_row_path", 0)
+
+
+@pytest.mark.timeout(600)
+def test_random_bfs(orc, cuda):
+    """Batched BFS over random symmetric graphs: tile dim, source count (1-64),
+    hipGraph or eager level loop, pull-only or direction switching; level
+    arrays identical to the oracle's queue BFS from every source."""
+    from bgb import b2sr as B
+    done = 0
+    for seed in range(7000, 7040):
+        rng = np.random.default_rng(seed)
+        label, csr = random_graph(rng)
+        if csr.nrows != csr.ncols:  # symmetric shapes only (push = A)
+            continue
+        T = int(rng.choice((4, 8, 16, 32)))
+        A = B.B2SR.from_csr(csr, T, cuda)
+        src = rng.integers(0, csr.nrows, int(rng.integers(1, 65)))
+        use_graph = bool(rng.integers(0, 2))
+        push = A if rng.integers(0, 2) else None
+        lev, nl = B.bfs_multi(A, src, use_graph, push=push)
+        want = np.stack([orc.bfs(csr.rowptr, csr.colind, int(s))[0] for s in src])
+        np.testing.assert_array_equal(lev.cpu().numpy(), want, err_msg=f"seed {seed} {label} T={T}")
+        assert nl == want.max() + 1
+        done += 1
+    assert done >= 25
+
+
+@pytest.mark.timeout(600)
+def test_random_k16(orc, cuda):
+    """K=16 MFMA bin_full_full over random graphs at T = 16 / 32, one launch per
+    graph and all of them in one persistent batch: within the FP32-accumulation
+    bound of the FP64 SpMM (1e-6 of sum |A||X| per entry, plus 1e-7)."""
+    import torch
+    from bgb import b2sr as B
+
+    def check(csr, Y, X64, label):
+        want = orc.csr_spmm_f64(csr.rowptr, csr.colind, X64)
+        bound = orc.csr_spmm_f64(csr.rowptr, csr.colind, np.abs(X64))
+        err = np.abs(Y[: csr.nrows].cpu().numpy().astype(np.float64) - want)
+        assert np.all(err <= 1e-6 * bound + 1e-7), label
+
+    mats, xbs, refs = [], [], []
+    for seed in range(8000, 8030):
+        rng = np.random.default_rng(seed)
+        label, csr = random_graph(rng)
+        if csr.nrows != csr.ncols:
+            continue
+        T = int(rng.choice((16, 32)))
+        A = B.B2SR.from_csr(csr, T, cuda)
+        X = torch.from_numpy(rng.uniform(-1, 1, (csr.nrows, 16)).astype(np.float16)).to(cuda)
+        X64 = X.cpu().numpy().astype(np.float64)
+        Xb = B.k16_pack_rhs(X, A.nblockrows * T)
+        check(csr, B.bmv_k16(A, Xb), X64, f"seed {seed} {label} T={T}")
+        mats.append(A)
+        xbs.append(Xb)
+        refs.append((csr, X64))
+    assert len(refs) >= 20
+    batch = B.K16Batch(mats, xbs, nworkgroups=96)
+    batch.run()
+    for i, (csr, X64) in enumerate(refs):
+        check(csr, batch.result(i), X64, f"batch graph {i}")
