@@ -284,8 +284,8 @@ struct K16BatchArgs {
   const int *tdim;
   const long long *chunk_start;  // [ngraphs + 1] prefix of chunk counts
   int ngraphs;
-  long long total;
-  unsigned long long *queue;  // next chunk index
+  long long cbegin, cend;      // this launch's chunks: all of one tile dim
+  unsigned long long *queue;  // next chunk index, relative to cbegin
   int *carry_key;
   float *carry_val;
   int *graph_of;
@@ -296,23 +296,27 @@ struct K16BatchArgs {
 // workgroup's waves through LDS between two barriers, so every wave runs the
 // loop in uniform control flow (no lane-0-only region feeds a wave-wide
 // shuffle) and the whole workgroup leaves the loop together once the queue
-// is drained.
+// is drained. Graphs are ordered by tile dim and each tile dim gets its own
+// launch (its own register budget: the T = 16 walk needs far fewer VGPRs
+// than the T = 32 one, so a shared kernel ran T = 16 graphs at T = 32's
+// occupancy).
+template <int T>
 __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArgs a) {
   __shared__ unsigned long long s_ticket;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_ticket = atomicAdd(a.queue, (unsigned long long)kWaves);
     __syncthreads();
-    const unsigned long long base = s_ticket;
+    const unsigned long long base = s_ticket + (unsigned long long)a.cbegin;
     __syncthreads();  // every thread has read the ticket before the next one is taken
-    if (base >= (unsigned long long)a.total) break;  // uniform per workgroup
+    if (base >= (unsigned long long)a.cend) break;  // uniform per workgroup
     // the chunk and its graph are wave-uniform: kept in scalar registers, so the
     // graph's pointers feed the buffer resources of the X gathers directly
     const unsigned long long c =
         ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32 |
          (unsigned)__builtin_amdgcn_readfirstlane((int)base)) +
         (unsigned)__builtin_amdgcn_readfirstlane(wave);
-    if (c < (unsigned long long)a.total) {  // uniform per wave
+    if (c < (unsigned long long)a.cend) {  // uniform per wave
       int lo = 0, hi = a.ngraphs - 1;  // graph with chunk_start[g] <= c < chunk_start[g+1]
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -324,8 +328,7 @@ __global__ __launch_bounds__(64 * kWaves) void k16_persistent_kernel(K16BatchArg
       const long long k0 = ((long long)c - a.chunk_start[lo]) * kChunk;
       const long long k1 = k0 + kChunk < G.nblocks ? k0 + kChunk : G.nblocks;
       float *carry = a.carry_val + (long long)c * 32 * kK;
-      const int open = a.tdim[lo] == 32 ? k16_chunk<32>(G, k0, k1, lane, carry)
-                                        : k16_chunk<16>(G, k0, k1, lane, carry);
+      const int open = k16_chunk<T>(G, k0, k1, lane, carry);
       // written by every lane (same value): no lane-0-only region in the loop
       a.carry_key[c] = open < G.nbr ? open : -1;
       a.graph_of[c] = lo;
@@ -422,19 +425,27 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
   std::vector<K16Graph> gs(ngraphs);
   std::vector<int> td(ngraphs);
   std::vector<long long> cs(ngraphs + 1, 0);
-  for (int i = 0; i < ngraphs; ++i) {
+  // T = 16 graphs first, then T = 32: one contiguous chunk range per tile dim
+  std::vector<int> order;
+  order.reserve(ngraphs);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < ngraphs; ++i)
+      if ((h_graphs[i].T == 32) == (pass == 1)) order.push_back(i);
+  int n16 = 0;
+  for (int i = 0; i < ngraphs; ++i) n16 += h_graphs[i].T != 32;
+  for (int slot = 0; slot < ngraphs; ++slot) {
+    const int i = order[slot];
     const bgb_k16_graph &h = h_graphs[i];
     BGB_REQUIRE(h.T == 16 || h.T == 32, BGB_ERR_TILEDIM, "graph %d: tile dim %d not in {16, 32}", i, h.T);
     BGB_REQUIRE(h.n >= 1 && h.nblocks >= 0 && h.val_byte_off % 16 == 0 && h.x_off % 256 == 0, BGB_ERR_ARG,
                 "graph %d: bad sizes or misaligned offsets", i);
     const int nbr = (int)ceil_div(h.n, h.T);
-    gs[i] = K16Graph{d_rowptr + h.rowptr_off, d_colind + h.tile_off,
-                     static_cast<const char *>(d_val) + h.val_byte_off,
-                     reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(d_Xb) + h.x_off),
-                     d_Y + h.y_off, nbr, h.nblocks, 0};
-    td[i] = h.T;
-    // a graph without tiles still owns one chunk, which writes its zero rows
-    cs[i + 1] = cs[i] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
+    gs[slot] = K16Graph{d_rowptr + h.rowptr_off, d_colind + h.tile_off,
+                        static_cast<const char *>(d_val) + h.val_byte_off,
+                        reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(d_Xb) + h.x_off),
+                        d_Y + h.y_off, nbr, h.nblocks, 0};
+    td[slot] = h.T;
+    cs[slot + 1] = cs[slot] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
     if (h.nblocks == 0)
       k16_fill_kernel<float><<<nb256((long long)nbr * h.T * kK), 256, 0, s>>>(d_Y + h.y_off, (long long)nbr * h.T * kK, 0.f);
   }
@@ -443,22 +454,28 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
   BGB_HIP_TRY(dg.alloc(sizeof(K16Graph) * ngraphs, s));
   BGB_HIP_TRY(dt.alloc(sizeof(int) * ngraphs, s));
   BGB_HIP_TRY(dc.alloc(sizeof(long long) * (ngraphs + 1), s));
-  BGB_HIP_TRY(q.alloc(sizeof(unsigned long long), s));
+  BGB_HIP_TRY(q.alloc(2 * sizeof(unsigned long long), s));  // one queue per tile dim
   BGB_HIP_TRY(ck.alloc(sizeof(int) * (total + 1), s));
   BGB_HIP_TRY(cv.alloc(sizeof(float) * 32 * kK * (total + 1), s));
   BGB_HIP_TRY(go.alloc(sizeof(int) * (total + 1), s));
   BGB_HIP_TRY(hipMemcpyAsync(dg.ptr, gs.data(), sizeof(K16Graph) * ngraphs, hipMemcpyHostToDevice, s));
   BGB_HIP_TRY(hipMemcpyAsync(dt.ptr, td.data(), sizeof(int) * ngraphs, hipMemcpyHostToDevice, s));
   BGB_HIP_TRY(hipMemcpyAsync(dc.ptr, cs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
-  k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 1, 0ull);
+  k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 2, 0ull);
   BGB_HIP_TRY(hipGetLastError());
   if (total > 0) {
-    K16BatchArgs a{dg.as<K16Graph>(), dt.as<int>(), dc.as<long long>(), ngraphs, total, q.as<unsigned long long>(),
-                   ck.as<int>(), cv.as<float>(), go.as<int>()};
-    int wgs = nworkgroups > 0 ? nworkgroups : 2048;
-    if ((long long)wgs * kWaves > total) wgs = (int)ceil_div(total, kWaves);
-    k16_persistent_kernel<<<wgs, 64 * kWaves, 0, s>>>(a);
-    BGB_HIP_TRY(hipGetLastError());
+    const long long split = cs[n16];  // chunks of the T = 16 graphs
+    for (int t = 0; t < 2; ++t) {
+      const long long cb = t == 0 ? 0 : split, ce = t == 0 ? split : total;
+      if (ce <= cb) continue;
+      K16BatchArgs a{dg.as<K16Graph>(), dt.as<int>(), dc.as<long long>(), ngraphs, cb, ce,
+                     q.as<unsigned long long>() + t, ck.as<int>(), cv.as<float>(), go.as<int>()};
+      int wgs = nworkgroups > 0 ? nworkgroups : 2048;
+      if ((long long)wgs * kWaves > ce - cb) wgs = (int)ceil_div(ce - cb, kWaves);
+      if (t == 0) k16_persistent_kernel<16><<<wgs, 64 * kWaves, 0, s>>>(a);
+      else k16_persistent_kernel<32><<<wgs, 64 * kWaves, 0, s>>>(a);
+      BGB_HIP_TRY(hipGetLastError());
+    }
     K16Graph none{};
     k16_fixup_kernel<<<nb256(ceil_div(total, kFixChunks) * 64), 256, 0, s>>>(ck.as<int>(), cv.as<float>(), total, dg.as<K16Graph>(),
                                                            go.as<int>(), dt.as<int>(), none, 0);
