@@ -469,18 +469,21 @@ __global__ void bfs_expand_kernel(const uint64_t *V, const uint64_t *P, long lon
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t vis = V[v];
-  uint64_t pl[kPlanes];
+  // plane-outer over the planes in use (ceil(log2(levels)), not all kPlanes):
+  // 64 level registers, one bit extract and OR per (plane, source)
+  int lev[64];
 #pragma unroll
-  for (int k = 0; k < kPlanes; ++k) pl[k] = k < nplanes ? P[k * npad + v] : 0;
-  for (int s = 0; s < nsrc; ++s) {
-    int lev = -1;
-    if ((vis >> s) & 1) {
-      lev = 0;
+  for (int s = 0; s < 64; ++s) lev[s] = 0;
+  for (int k = 0; k < nplanes; ++k) {
+    const uint64_t pk = P[k * npad + v];
+    const uint32_t lo = (uint32_t)pk, hi = (uint32_t)(pk >> 32);
 #pragma unroll
-      for (int k = 0; k < kPlanes; ++k) lev |= (int)((pl[k] >> s) & 1) << k;
-    }
-    level[(long long)s * n + v] = lev;
+    for (int s = 0; s < 64; ++s)
+      lev[s] |= (int)(((s < 32 ? lo >> s : hi >> (s - 32)) & 1u) << k);
   }
+#pragma unroll
+  for (int s = 0; s < 64; ++s)
+    if (s < nsrc) level[(long long)s * n + v] = ((vis >> s) & 1) ? lev[s] : -1;
 }
 
 // ---- plain per-source loop ----
