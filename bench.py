@@ -207,6 +207,45 @@ def rocsparse_baseline(torch, csr, x01, reps=20):
     return out
 
 
+def er4096_section(torch, B, graph, steps=200, warmup=20, with_cpu=True):
+    """bin_bin_bin BMV on Erdos-Renyi n=4096, average degree 16 (65,536 nnz), seed 1,
+    T=32, x Bernoulli(0.1): time, GTEPS and bit-exact parity with the oracle's CSR
+    OR-AND loop. The working set (~2 MB) stays in cache, so no HBM fraction."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    csr = graph.erdos_renyi(4096, 16, seed=1)
+    T = 32
+    A = B.B2SR.from_csr(csr, T, dev)
+    x01 = graph.bernoulli(csr.nrows, 0.1, 1)
+    xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+    C = torch.empty(A.nblockrows, dtype=B.WORD_TORCH[T], device=dev)
+    t, _ = time_bmv(torch, lambda: B.bmv_bin_bin_bin(A, xb, out=C), steps, warmup)
+    out = {"n": csr.nrows, "nnz": csr.nnz, "tile_dim": T, "nblocks": A.nblocks,
+           "ms": round(t * 1e3, 5), "gteps": round(csr.nnz / t / 1e9, 3)}
+    if with_cpu:
+        import oracle_lib
+        orc = oracle_lib.Oracle()
+        want = orc.pack_u8(T, orc.csr_or(csr.rowptr, csr.colind, x01.astype(np.uint8)), A.nblockrows)
+        out["bit_exact_vs_oracle"] = bool(np.array_equal(C.cpu().numpy().view(np.uint32), want))
+        tc, reps = timed_reps(lambda: orc.csr_or(csr.rowptr, csr.colind, x01.astype(np.uint8)), 2.0)
+        out["cpu_baseline"] = {"value": round(csr.nnz / tc / 1e9, 3), "unit": "GTEPS", "cores": orc.threads(),
+                               "kind": "port", "sample": f"oracle CSR OR-AND loop, the full graph, {reps} reps"}
+    log(f"[bench] ER-4096 bin_bin_bin T=32: {out}")
+    del A, xb, C
+    return out
+
+
+def bfs_pull_bytes(T, nbr, nblocks):
+    """Algorithmic bytes of one pull level of the batched BFS: the tiles and block
+    columns, rowptr, the u64 frontier (each word counted once), visited and next
+    frontier words (8 B per vertex each), and the packed frontier-non-empty vector
+    read and written."""
+    wb = {4: 1, 8: 1, 16: 2, 32: 4}[T]
+    npad = nbr * T
+    nwords = -(-nbr // T) * T
+    return nblocks * (4 + T * wb) + 4 * (nbr + 1) + 3 * 8 * npad + 2 * wb * nwords
+
+
 def bfs_section(torch, B, graph, csr, T, nsrc=64, reps=3, plain=True):
     """Batched 64-source BFS (hipGraph level loop) vs the per-source masked-BMV loop."""
     import numpy as np
@@ -240,6 +279,26 @@ def bfs_section(torch, B, graph, csr, T, nsrc=64, reps=3, plain=True):
            "pull_only_identical": bool(torch.equal(lev_pull, lev))}
     log(f"[bench] BFS batched T={T}: {t_batch * 1e3:.3f} ms, {nl} levels, {edges / t_batch / 1e9:.2f} GTEPS "
         f"(pull only {t_pull * 1e3:.3f} ms)")
+    # per level, with the eager level loop between HIP events: the pull levels
+    # against the HBM roofline on their algorithmic bytes
+    lev_t, _, levels = B.bfs_level_times(A, src, push=A)
+    pb = bfs_pull_bytes(T, A.nblockrows, A.nblocks)
+    per_level = []
+    for i, (ms, pulled) in enumerate(levels):
+        e = {"level": i, "ms": round(ms, 4), "pass": "pull" if pulled else "push"}
+        if pulled:
+            e.update({"alg_bytes": pb, "alg_gbs": round(pb / (ms / 1e3) / 1e9, 1),
+                      "hbm_frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+        per_level.append(e)
+    out["per_level"] = per_level
+    out["per_level_identical"] = bool(torch.equal(lev_t, lev))
+    pulls = [e for e in per_level if e["pass"] == "pull"]
+    if pulls:
+        out["pull_roofline"] = {"bound": "hbm", "alg_bytes_per_level": pb,
+                                "best_frac": max(e["hbm_frac"] for e in pulls),
+                                "mean_ms": round(sum(e["ms"] for e in pulls) / len(pulls), 4)}
+    log(f"[bench] BFS per level: {per_level}")
+    del lev_t
     if plain:
         plv = torch.empty((nsrc, csr.nrows), dtype=torch.int32, device=dev)
         B.bfs_plain(A, int(src[0]), out=plv[0])  # warm-up
@@ -285,7 +344,9 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
     T = T or tuned
     A = B.B2SR.from_csr(csr, T, dev)
     out = {"n": csr.nrows, "nnz": csr.nnz, "autotuned_tile_dim": tuned, "pagerank_tile_dim": T,
-           "bin_full_full_per_tile": per_tile}
+           "bin_full_full_per_tile": per_tile,
+           "rocsparse_csrmv": rocsparse_baseline(torch, csr, x)}
+    log(f"[bench] grid rocSPARSE csrmv: {out['rocsparse_csrmv']}")
     ref = None
     if with_cpu:
         # CPU baseline of this section: the oracle's FP64 power iteration
@@ -299,13 +360,20 @@ def pagerank_section(torch, B, graph, T=None, side=4096, iters=50, with_cpu=True
                                "cores": orc.threads(), "kind": "port",
                                "sample": f"oracle FP64 PageRank, the full {side}^2 grid, {iters} iterations"}
     modes = {}
+    s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for mode in ("fp32", "bf16", "fp64sum"):
         B.pagerank(A, 0.85, 2, mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        s_ev.record()
         pr, res = B.pagerank(A, 0.85, iters, mode)
+        e_ev.record()
+        torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        m = {"ms": round(el * 1e3, 2), "final_residual": float(res[-1])}
+        # GPU time between HIP events on the call's stream (its setup -- degrees,
+        # buffers -- included), and the host wall time of the call
+        m = {"ms": round(s_ev.elapsed_time(e_ev), 3), "wall_ms": round(el * 1e3, 2),
+             "final_residual": float(res[-1])}
         if ref is not None:
             p = pr.cpu().numpy().astype(np.float64)
             m["max_rel_err_vs_fp64"] = float(np.max(np.abs(p - ref) / ref))
@@ -373,17 +441,13 @@ def k16batch_section(torch, B, graph, ngraphs=1000, nmin=1e3, nmax=1e6, batch_by
     |A_vu| |X_u|) over entries (the FP32-accumulation bound)."""
     import numpy as np
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(seed)
     gen_t = torch.Generator(device=dev)
     gen_t.manual_seed(seed)
     orc = None
     if with_cpu:
         import oracle_lib
         orc = oracle_lib.Oracle()
-    specs = []
-    for i in range(ngraphs):
-        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
-        specs.append(("er", n, int(rng.integers(8, 33))) if i % 2 == 0 else ("rmat", n, 8))
+    specs = graph.uneven_graph_specs(ngraphs, nmin, nmax, seed)
     tot = {"graphs": 0, "n": 0, "nnz": 0, "batches": 0, "gpu_s": 0.0, "cpu_s": 0.0, "gen_s": 0.0}
     tdims = {16: 0, 32: 0}
     errs = []
@@ -418,12 +482,9 @@ def k16batch_section(torch, B, graph, ngraphs=1000, nmin=1e3, nmax=1e6, batch_by
         pending, pending_bytes = [], 0
         torch.cuda.empty_cache()
 
-    for i, (kind, n, deg) in enumerate(specs):
+    for i, spec in enumerate(specs):
         t0 = time.perf_counter()
-        if kind == "er":
-            csr = graph.erdos_renyi(n, deg, seed=1000 + i)
-        else:
-            csr = graph.rmat(max(10, int(round(np.log2(n)))), deg, seed=1000 + i)
+        csr = graph.uneven_graph(spec)
         tot["gen_s"] += time.perf_counter() - t0
         T, _ = graph.autotune_tiledim(csr, "k16", allowed=(16, 32))
         A = B.B2SR.from_csr(csr, T, dev)
@@ -467,8 +528,9 @@ def main():
     ap.add_argument("--config", default="rmat20", choices=sorted(CONFIGS))
     ap.add_argument("--tiles", default="4,8,16,32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--extras", default="rmat22,bfs,pagerank,k16",
-                    help="comma list of sections beside the R-MAT 20 sweep: rmat22 (bin_bin_full on R-MAT "
+    ap.add_argument("--extras", default="er4096,rmat22,bfs,pagerank,k16,k16batch",
+                    help="comma list of sections beside the R-MAT 20 sweep: er4096 (bin_bin_bin on "
+                         "Erdos-Renyi n=4096, T=32), rmat22 (bin_bin_full on R-MAT "
                          "scale 22), bfs (batched 64-source BFS vs the plain per-source loop on R-MAT 22), "
                          "pagerank (4096^2 grid bin_full_full + PageRank in three precisions), k16 (K=16 MFMA "
                          "bin_full_full on R-MAT 20), k16batch "
@@ -598,6 +660,8 @@ def main():
 
     extras = {}
     wanted = [w for w in args.extras.split(",") if w and w != "none"]
+    if "er4096" in wanted:
+        extras["bin_bin_bin_er4096"] = er4096_section(torch, B, graph, with_cpu=not args.no_cpu)
     if "rmat22" in wanted or "bfs" in wanted:
         log("[bench] generating R-MAT scale 22 ef 16")
         t0 = time.perf_counter()
@@ -606,19 +670,27 @@ def main():
         if "rmat22" in wanted:
             x22 = graph.bernoulli(csr22.nrows, 0.05, 2)
             pt22 = {}
+            flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
             for T in tiles:
                 A = B.B2SR.from_csr(csr22, T, dev)
                 xb = B.pack_vec(B.padded_vector(x22, T, dev), T)
                 C = torch.empty(A.nblockrows * T, dtype=torch.int32, device=dev)
-                t, _ = time_bmv(torch, lambda: B.bmv_bin_bin_full(A, xb, out=C), args.steps, args.warmup)
+                fn = lambda: B.bmv_bin_bin_full(A, xb, out=C)  # noqa: E731
+                t, _ = time_bmv(torch, fn, args.steps, args.warmup)
+                tcold = time_cold(torch, fn, flush)
                 nbytes = bmv_bytes(T, A.nblockrows, A.nblocks)
                 pt22[str(T)] = {"ms": round(t * 1e3, 4), "gteps": round(csr22.nnz / t / 1e9, 2),
                                 "alg_gbs": round(nbytes / t / 1e9, 1),
-                                "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "nblocks": A.nblocks}
+                                "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "nblocks": A.nblocks,
+                                "cold_ms": round(tcold * 1e3, 4), "cold_gteps": round(csr22.nnz / tcold / 1e9, 2),
+                                "cold_hbm_frac": round(nbytes / tcold / 1e9 / HBM_PEAK_GBS, 4)}
                 log(f"[bench] rmat22 T={T}: {pt22[str(T)]}")
                 del A, xb, C
                 torch.cuda.empty_cache()
-            extras["bin_bin_full_rmat22"] = {"nnz": csr22.nnz, "per_tile": pt22}
+            del flush
+            rs22 = rocsparse_baseline(torch, csr22, x22)
+            log(f"[bench] rmat22 rocSPARSE csrmv: {rs22}")
+            extras["bin_bin_full_rmat22"] = {"nnz": csr22.nnz, "per_tile": pt22, "rocsparse_csrmv": rs22}
         if "bfs" in wanted:
             bt = args.bfs_tile or graph.autotune_tiledim(csr22, "bfs")[0]
             extras["bfs_rmat22"] = bfs_section(torch, B, graph, csr22, bt)

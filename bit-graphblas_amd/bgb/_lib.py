@@ -64,6 +64,8 @@ SIGNATURES = {
                                _vp, _vp]),
     "bgb_bfs_multi_dir": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_int,
                                    _vp, _c_int, _vp, _vp]),
+    "bgb_set_bfs_level_timing": (_c_int, [_c_int]),
+    "bgb_bfs_level_times": (_c_int, [_c_int, _vp, _vp, _vp]),
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_autotune_tiledim": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp]),

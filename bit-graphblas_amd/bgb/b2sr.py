@@ -139,13 +139,8 @@ def _out(out, n, dtype, device, accumulate):
 
 
 def _bits_out(out, A):
-    """Bit-output words; fresh buffers are padded to whole dwords (pieces of rows
-    that cross chunks are OR-ed into the aligned dword that holds their word)."""
-    if out is not None:
-        return _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, True)
-    wb = lib().bgb_word_bytes(A.T)
-    words = -(-(A.nblockrows * wb) // 4) * 4 // wb
-    return torch.zeros(max(words, 1), dtype=WORD_TORCH[A.T], device=A.device)[: A.nblockrows]
+    """Bit-output words: exactly NBR of them (nothing past the last is touched)."""
+    return _out(out, A.nblockrows, WORD_TORCH[A.T], A.device, True)
 
 
 def bmv_bin_bin_bin(A, xb, out=None, accumulate=False):
@@ -214,6 +209,23 @@ def bfs_multi(A, sources, use_graph=True, push=None):
              _ptr(push.val), _ptr(push.rowptr), _ptr(push.colind), n, src.ctypes.data, src.size, _ptr(lev),
              int(use_graph), ctypes.byref(nl), _stream())
     return lev, nl.value
+
+
+def bfs_level_times(A, sources, push=None):
+    """Runs bfs_multi with the eager level loop, one level at a time between HIP
+    events: returns (levels, nlevels, [(ms, pulled)] per level)."""
+    lib().bgb_set_bfs_level_timing(1)
+    try:
+        lev, nl = bfs_multi(A, sources, use_graph=False, push=push)
+    finally:
+        lib().bgb_set_bfs_level_timing(0)
+    cap = 1 << 16
+    ms = np.zeros(cap, np.float32)
+    pulled = np.zeros(cap, np.int32)
+    n = ctypes.c_int(0)
+    call("bgb_bfs_level_times", cap, ms.ctypes.data, pulled.ctypes.data, ctypes.byref(n))
+    k = min(n.value, cap)
+    return lev, nl, [(float(ms[i]), bool(pulled[i])) for i in range(k)]
 
 
 def bfs_plain(A, source, out=None):

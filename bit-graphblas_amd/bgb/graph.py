@@ -122,3 +122,27 @@ def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):
     t = ctypes.c_int(0)
     call("bgb_autotune_tiledim", OP_FAMILY[op], csr.nrows, nb.ctypes.data, mask, ctypes.byref(t), cost.ctypes.data)
     return t.value, {T: float(cost[i]) for i, T in enumerate((4, 8, 16, 32))}
+
+
+def uneven_graph_specs(ngraphs=1000, nmin=1e3, nmax=1e6, seed=5):
+    """Specs of a batch of graphs of uneven size: n log-uniform in [nmin, nmax];
+    even indices Erdos-Renyi with average degree U{8..32}, odd ones R-MAT with
+    edge factor 8. Returns [(kind, n, degree or edge factor, graph seed)]."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(ngraphs):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        if i % 2 == 0:
+            specs.append(("er", n, int(rng.integers(8, 33)), 1000 + i))
+        else:
+            specs.append(("rmat", n, 8, 1000 + i))
+    return specs
+
+
+def uneven_graph(spec):
+    """The graph of one uneven_graph_specs entry (R-MAT at scale round(log2 n),
+    at least 10, so its n is the nearest power of two)."""
+    kind, n, deg, seed = spec
+    if kind == "er":
+        return erdos_renyi(n, deg, seed=seed)
+    return rmat(max(10, int(round(np.log2(n)))), deg, seed=seed)

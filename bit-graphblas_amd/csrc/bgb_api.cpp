@@ -9,7 +9,7 @@ const char *bgb_status_string(int status) {
   switch (status) {
     case BGB_OK: return "ok";
     case BGB_ERR_ARG: return "invalid argument";
-    case BGB_ERR_TILEDIM: return "tile dim not in {4, 8, 16, 32}";
+    case BGB_ERR_TILEDIM: return "tile dim not in {4, 8, 16, 32, 64} or not taken by this entry point";
     case BGB_ERR_HIP: return "HIP runtime error";
     case BGB_ERR_ALLOC: return "allocation failed";
     case BGB_ERR_SIZE: return "size out of range";

@@ -515,6 +515,29 @@ __global__ void plain_seed_kernel(int src, void *front, void *visited, int *leve
 }
 
 unsigned nblk(long long n, int b = 256) { return (unsigned)ceil_div(n > 0 ? n : 1, b); }
+
+// Per-level timing of the eager level loop (bgb_set_bfs_level_timing): the
+// GPU time of each level of the last timed call and whether it pulled.
+bool g_level_timing = false;
+struct LevelTimes {
+  std::vector<float> ms;
+  std::vector<int> pulled;
+};
+LevelTimes &level_times() {
+  static thread_local LevelTimes t;
+  return t;
+}
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&a);
+    return e != hipSuccess ? e : hipEventCreate(&b);
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
 // grids of the grid-stride fills (up to 16 words per vertex)
 unsigned nfill(long long n) { return capped_blocks(n, 256); }
 
@@ -595,20 +618,21 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 8192);
   const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
   const long long zbytes = wb * nwords + 16;
-  auto level_pair = [&]() {
-    for (int p = 0; p < 2; ++p) {
-      bfs_clear_kernel<<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
-      if (nchunks > 0) {
-        bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
-        bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
-        if (push) {
-          bfs_push_kernel<T><<<pblocks, 256, 0, s>>>(st[p]);
-          bfs_push_long_kernel<T><<<1024, 256, 0, s>>>(st[p]);
-          bfs_push_finish_kernel<T><<<nblk(nbr), 256, 0, s>>>(st[p]);
-        }
+  auto one_level = [&](int p) {
+    bfs_clear_kernel<<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
+    if (nchunks > 0) {
+      bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
+      bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
+      if (push) {
+        bfs_push_kernel<T><<<pblocks, 256, 0, s>>>(st[p]);
+        bfs_push_long_kernel<T><<<1024, 256, 0, s>>>(st[p]);
+        bfs_push_finish_kernel<T><<<nblk(nbr), 256, 0, s>>>(st[p]);
       }
-      bfs_advance_kernel<<<1, kCounters, 0, s>>>(ctl.as<int>(), fc.as<unsigned long long>(), push_limit);
     }
+    bfs_advance_kernel<<<1, kCounters, 0, s>>>(ctl.as<int>(), fc.as<unsigned long long>(), push_limit);
+  };
+  auto level_pair = [&]() {
+    for (int p = 0; p < 2; ++p) one_level(p);
   };
   int h_ctl[4] = {0, 0, 0, 0};
   const int max_pairs = (1 << kPlanes) / 2 - 1;
@@ -660,6 +684,30 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     (void)hipStreamSynchronize(cs);
     (void)hipStreamDestroy(cs);
     if (st_code) return set_error(st_code, "BFS level graph: %s failed", what);
+  } else if (g_level_timing) {
+    // one level at a time, each between two events; the flags read before a
+    // level say whether it pushes
+    LevelTimes &lt = level_times();
+    lt.ms.clear();
+    lt.pulled.clear();
+    EventPair ev;
+    BGB_HIP_TRY(ev.create());
+    for (int lv = 0; lv < 2 * max_pairs; ++lv) {
+      BGB_HIP_TRY(hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, s));
+      BGB_HIP_TRY(hipStreamSynchronize(s));
+      if (h_ctl[2]) break;
+      BGB_HIP_TRY(hipEventRecord(ev.a, s));
+      one_level(lv & 1);
+      BGB_HIP_TRY(hipGetLastError());
+      BGB_HIP_TRY(hipEventRecord(ev.b, s));
+      BGB_HIP_TRY(hipEventSynchronize(ev.b));
+      float ms = 0.f;
+      BGB_HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+      lt.ms.push_back(ms);
+      lt.pulled.push_back(h_ctl[3] ? 0 : 1);
+    }
+    BGB_HIP_TRY(hipMemcpyAsync(h_ctl, ctl.ptr, sizeof(h_ctl), hipMemcpyDeviceToHost, s));
+    BGB_HIP_TRY(hipStreamSynchronize(s));
   } else {
     for (int pairs = 0; pairs < max_pairs; ++pairs) {
       level_pair();
@@ -739,43 +787,69 @@ extern "C" {
 int bgb_bfs_multi(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
                   int nblocks, int n, const int *h_sources, int nsrc, int *d_levels, int use_graph,
                   int *h_nlevels, bgb_stream_t stream) {
-  return bgb_bfs_multi_dir(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, nullptr, nullptr, nullptr, n,
-                           h_sources, nsrc, d_levels, use_graph, h_nlevels, stream);
+  return host_guard([&]() -> int {
+    return bgb_bfs_multi_dir(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, nullptr, nullptr, nullptr, n,
+                             h_sources, nsrc, d_levels, use_graph, h_nlevels, stream);
+  });
 }
 
 int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
                       int nblocks, const void *d_At, const int *d_rowptrT, const int *d_colindT, int n,
                       const int *h_sources, int nsrc, int *d_levels, int use_graph, int *h_nlevels,
                       bgb_stream_t stream) {
+  return host_guard([&]() -> int {
+    clear_error();
+    int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_levels, h_nlevels);
+    if (st) return st;
+    BGB_REQUIRE(h_sources && nsrc >= 1 && nsrc <= 64, BGB_ERR_ARG, "need 1..64 sources");
+    for (int i = 0; i < nsrc; ++i)
+      BGB_REQUIRE(h_sources[i] >= 0 && h_sources[i] < n, BGB_ERR_ARG, "source %d out of range", h_sources[i]);
+    BGB_REQUIRE((d_At == nullptr) == (d_rowptrT == nullptr) && (d_At == nullptr) == (d_colindT == nullptr),
+                BGB_ERR_ARG, "pass all of At, rowptrT, colindT, or none");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (T) {
+      case 4: return bfs_multi_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+      case 8: return bfs_multi_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+      case 16: return bfs_multi_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+      default: return bfs_multi_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+    }
+  });
+}
+
+int bgb_set_bfs_level_timing(int on) {
   clear_error();
-  int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_levels, h_nlevels);
-  if (st) return st;
-  BGB_REQUIRE(h_sources && nsrc >= 1 && nsrc <= 64, BGB_ERR_ARG, "need 1..64 sources");
-  for (int i = 0; i < nsrc; ++i)
-    BGB_REQUIRE(h_sources[i] >= 0 && h_sources[i] < n, BGB_ERR_ARG, "source %d out of range", h_sources[i]);
-  BGB_REQUIRE((d_At == nullptr) == (d_rowptrT == nullptr) && (d_At == nullptr) == (d_colindT == nullptr),
-              BGB_ERR_ARG, "pass all of At, rowptrT, colindT, or none");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  switch (T) {
-    case 4: return bfs_multi_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    case 8: return bfs_multi_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    case 16: return bfs_multi_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
-    default: return bfs_multi_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, d_At, d_rowptrT, d_colindT, n, h_sources, nsrc, d_levels, use_graph, h_nlevels, s);
+  BGB_REQUIRE(on == 0 || on == 1, BGB_ERR_ARG, "on must be 0 or 1 (got %d)", on);
+  g_level_timing = on != 0;
+  return BGB_OK;
+}
+
+int bgb_bfs_level_times(int cap, float *h_ms, int *h_pulled, int *h_n) {
+  clear_error();
+  BGB_REQUIRE(cap >= 0 && h_n && (cap == 0 || (h_ms && h_pulled)), BGB_ERR_ARG, "bad arguments");
+  const LevelTimes &lt = level_times();
+  const int n = (int)std::min<size_t>(lt.ms.size(), (size_t)cap);
+  for (int i = 0; i < n; ++i) {
+    h_ms[i] = lt.ms[i];
+    h_pulled[i] = lt.pulled[i];
   }
+  *h_n = (int)lt.ms.size();
+  return BGB_OK;
 }
 
 int bgb_bfs_plain(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
                   int nblocks, int n, int source, int *d_level, int *h_nlevels, bgb_stream_t stream) {
-  clear_error();
-  int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_level, h_nlevels);
-  if (st) return st;
-  BGB_REQUIRE(source >= 0 && source < n, BGB_ERR_ARG, "source out of range");
-  switch (T) {
-    case 4: return bfs_plain_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
-    case 8: return bfs_plain_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
-    case 16: return bfs_plain_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
-    default: return bfs_plain_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
-  }
+  return host_guard([&]() -> int {
+    clear_error();
+    int st = bfs_check(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, d_level, h_nlevels);
+    if (st) return st;
+    BGB_REQUIRE(source >= 0 && source < n, BGB_ERR_ARG, "source out of range");
+    switch (T) {
+      case 4: return bfs_plain_impl<4>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+      case 8: return bfs_plain_impl<8>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+      case 16: return bfs_plain_impl<16>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+      default: return bfs_plain_impl<32>(d_A, d_rowptr, d_colind, nblockrows, nblocks, n, source, d_level, h_nlevels, stream);
+    }
+  });
 }
 
 }  // extern "C"

@@ -312,7 +312,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
           for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
           open = nz;
         }
-        if (q == 0) a.carry_key[chunk] = open ? rw.Ib : -1;
+        if (q == 0) a.carry_key[chunk] = open ? rw.Ib : -2 - rw.Ib;
         if (open) {
           Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
 #pragma unroll
@@ -470,9 +470,10 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
     for (int i = 0; i < R; ++i) hv[i] += cin[i];
     flush_row<T, OP>(a, held, q, hv);
   }
-  // The last active group publishes the chunk's open row. Bit carries are
-  // skipped when all zero (key -1); count / FP carries are always published so
-  // that the chunks carrying into one row stay contiguous for the fix-up.
+  // The last active group publishes the chunk's open row I (nbr: none). Bit
+  // carries that are all zero publish no record (key -2 - I, so the row stays
+  // known to the fix-up's tail wave); count / FP carries are always published
+  // so that the chunks carrying into one row stay contiguous for the fix-up.
   const int last = (int)lmin(G - 1, (a.nblocks - 1 - k0) / S);
   if (g == last) {
     bool open = key < a.nbr;
@@ -484,7 +485,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
       for (int m = 1; m < L; m <<= 1) nz |= __shfl_xor((int)nz, m) != 0;
       open = open && nz;
     }
-    if (q == 0) a.carry_key[chunk] = open ? key : -1;
+    if (q == 0) a.carry_key[chunk] = open ? key : -2 - key;
     if (open) {
       Acc *cv = static_cast<Acc *>(a.carry_val) + (long long)chunk * T + q * R;
 #pragma unroll
@@ -609,15 +610,57 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
   }
 }
 
+// Bit carries of the output words in the last, partial dword of C (words
+// tail..nbr-1, at most three): the chunks carrying into them are the last
+// chunks (open rows never decrease over chunks), so one wave walks the carry
+// keys back from the last chunk, 64 at a time, until a chunk's open row lies
+// before `tail`, ORs each word's carries over the wave and merges them into C
+// with plain stores of those words alone -- no byte past C[nbr-1] is read or
+// written, so C may be exactly nbr words (the reference caller's allocation,
+// test_bin_bin_bin.cu:329). The other fix-up threads skip these rows.
+template <int T, int OP>
+__device__ __forceinline__ void bits_tail(const BmvArgs &a, int tail, int lane) {
+  using W = typename Tile<T>::W;
+  static_assert(sizeof(W) < 4, "only sub-dword words have a partial last dword");
+  uint32_t w[3] = {0u, 0u, 0u};
+  for (long long base = (long long)a.nchunks - 1; base >= 0; base -= 64) {
+    const long long c = base - lane;
+    const int k = c >= 0 ? a.carry_key[c] : -2;
+    const int row = k >= 0 ? k : -2 - k;
+    uint32_t bits = 0;
+    if (k >= tail) {  // a non-zero record of a tail row (k < nbr)
+      const uint32_t *cv = static_cast<const uint32_t *>(a.carry_val) + c * T;
+#pragma unroll
+      for (int r = 0; r < T; ++r) bits |= (cv[r] != 0u ? 1u : 0u) << (T - 1 - r);
+      if constexpr (OP == OP_MASKED) bits &= ~(uint32_t) static_cast<const W *>(a.mask)[k];
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) w[t] |= k - tail == t ? bits : 0u;
+    if (__ballot(c < 0 || row < tail)) break;  // earlier chunks end before the tail rows
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) w[t] |= (uint32_t)__shfl_xor((int)w[t], m);
+  if (lane == 0) {
+    W *C = static_cast<W *>(a.C);
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+      if (tail + t < a.nbr && w[t]) C[tail + t] = (W)((uint32_t)C[tail + t] | w[t]);
+  }
+}
+
 // Adds the chunk carries of rows that cross chunk boundaries after the main
 // pass.
 //
-// Bit carries (skipped when zero, key -1): one thread per chunk, one 32-bit OR
-// into the aligned dword holding the row's word.
+// Bit carries (no record when zero, key < 0): one thread per chunk, one
+// 32-bit OR into the aligned dword holding the row's word (a 64-bit OR at
+// T = 64); the words of a partial last dword are merged by bits_tail in the
+// grid's last workgroup.
 //
 // Count / FP carries: one wave per 64 consecutive chunks, lane j owning chunk
-// c0 + j. Carry keys are non-decreasing over chunks (-1 only past the last
-// block row), so the chunks carrying into one row are contiguous; a segmented
+// c0 + j. Carry keys are non-decreasing over chunks (negative only past the
+// last block row), so the chunks carrying into one row are contiguous; a segmented
 // scan over the lanes sums each run and the run's last lane adds the sum to
 // the output -- a plain read-modify-write when the run lies inside this
 // wave's 64 chunks (it is then the row's only writer in this pass), atomics
@@ -627,10 +670,17 @@ __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+    // first word whose aligned dword reaches past C[nbr-1] (nbr: none)
+    const int tail = sizeof(W) < 4 ? (int)((((long long)a.nbr * sizeof(W)) & ~3ll) / sizeof(W)) : a.nbr;
+    if (blockIdx.x == gridDim.x - 1) {  // the tail workgroup
+      if constexpr (sizeof(W) < 4)
+        if (threadIdx.x < 64 && tail < a.nbr) bits_tail<T, OP>(a, tail, threadIdx.x);
+      return;
+    }
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.nchunks) return;
     const int key = a.carry_key[tid];
-    if (key < 0) return;
+    if (key < 0 || key >= tail) return;  // no record, or a tail word
     const Acc *cv = static_cast<const Acc *>(a.carry_val) + tid * T;
     Word<T> bits = 0;
 #pragma unroll
@@ -762,6 +812,26 @@ __global__ __launch_bounds__(256) void max_row_tiles_kernel(const int *rowptr, i
   }
 }
 
+// Largest block column (max of colind[0..n)) into *out (pre-zeroed), same
+// shape as max_row_tiles_kernel.
+__global__ __launch_bounds__(256) void max_col_kernel(const int *colind, long long n, int *out) {
+  __shared__ int s_m[4];
+  int m = 0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    m = colind[k] > m ? colind[k] : m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int t = __shfl_xor(m, o);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int b = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+    if (b > 0) atomicMax(out, b);
+  }
+}
+
 template <class V>
 __global__ void fill_kernel(V *p, long long n, V v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -786,10 +856,16 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   bool lds_path = false;
   if constexpr (!is_full(OP)) {
     // Whole packed x in LDS when it fits and the tile stream is large next to
-    // staging it once per workgroup.
+    // staging it once per workgroup. The staged words cover every block
+    // column the matrix references (a wide matrix reads past ceil(NBR/T)*T).
     DeviceFacts f;
     if ((st = device_facts(&f))) return st;
-    const long long x_bytes = ceil_div(a.nbr, T) * T * (long long)sizeof(typename Tile<T>::W);
+    int maxcol = -1;
+    const int mode = g_x_staging[ilog2_tiledim(T) - 2];
+    if (mode != 1 && (mode == 2 || T <= 16))
+      if ((st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
+    const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
+    const long long x_bytes = words * (long long)sizeof(typename Tile<T>::W);
     const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;
     constexpr int kLdsWaves = lds_waves<T, S>();
     const long long lds = (long long)sizeof(int) * kLdsWaves * kWinAlloc + xl;
@@ -798,9 +874,8 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
     // automatic: T <= 16 only (T = 32 streams 132 B per tile and is
     // bandwidth-bound; the persistent grid's lower occupancy costs more there
     // than the x gathers it saves, measured on MI355X)
-    const int mode = g_x_staging[ilog2_tiledim(T) - 2];
     const bool worth = mode == 2 || (mode == 0 && T <= 16 && tile_bytes >= 2 * xl * f.cus);
-    if (mode != 1 && aligned && lds <= f.lds_per_block && worth) {
+    if (maxcol >= 0 && aligned && lds <= f.lds_per_block && worth) {
       static bool attr_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S>),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)f.lds_per_block) == hipSuccess;
@@ -820,9 +895,11 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
     bmv_kernel<T, OP, S><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
     BGB_HIP_TRY(hipGetLastError());
   }
-  // bits: one thread per chunk; count / FP: one wave per 64 chunks
-  const long long fix_threads = (OP == OP_BITS || OP == OP_MASKED) ? nchunks : ceil_div(nchunks, 64) * 64;
-  bmv_fixup_kernel<T, OP><<<(unsigned)ceil_div(fix_threads, 256), 256, 0, s>>>(a);
+  // bits: one thread per chunk, plus the tail workgroup; count / FP: one wave
+  // per 64 chunks
+  constexpr bool kBits = OP == OP_BITS || OP == OP_MASKED;
+  const long long fix_blocks = kBits ? ceil_div(nchunks, 256) + 1 : ceil_div(ceil_div(nchunks, 64) * 64, 256);
+  bmv_fixup_kernel<T, OP><<<(unsigned)fix_blocks, 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }
@@ -855,38 +932,50 @@ int launch_bmv(BmvArgs a, hipStream_t s) {
 template <int OP>
 int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, const int *colind,
               int nbr, int nblocks, int accumulate, const void *mask, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(nbr >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
-  BGB_REQUIRE(C && rowptr, BGB_ERR_ARG, "null output or rowptr");
-  BGB_REQUIRE(nblocks == 0 || (A && B && colind), BGB_ERR_ARG, "null matrix / vector");
-  BGB_REQUIRE(OP != OP_MASKED || mask, BGB_ERR_ARG, "null mask");
-  BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
-  BGB_REQUIRE((long long)nbr * T < INT_MAX, BGB_ERR_SIZE, "NBR*T exceeds the int32 range");
-  auto aligned16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  BGB_REQUIRE(aligned16(A) && aligned16(colind), BGB_ERR_ARG,
-              "bsrVal and bsrColInd must be 16-byte aligned");
-  BGB_REQUIRE(OP == OP_BITS || OP == OP_MASKED || aligned16(C), BGB_ERR_ARG,
-              "count / float outputs must be 16-byte aligned");
-  BGB_REQUIRE((reinterpret_cast<uintptr_t>(C) & 3) == 0, BGB_ERR_ARG, "outputs must be 4-byte aligned");
-  if (nbr == 0) return BGB_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (nblocks == 0) {  // every block row is empty
-    if (accumulate && OP != OP_MASKED) return BGB_OK;
-    const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)nbr * word_bytes(T)
-                                                           : (long long)nbr * T * 4;
-    fill_kernel<uint8_t><<<capped_blocks(n, 256), 256, 0, s>>>(static_cast<uint8_t *>(C), n, 0);
-    BGB_HIP_TRY(hipGetLastError());
-    return BGB_OK;
-  }
-  BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr};
-  switch (T) {
-    case 4: return launch_bmv<4, OP>(a, s);
-    case 8: return launch_bmv<8, OP>(a, s);
-    case 16: return launch_bmv<16, OP>(a, s);
-    case 32: return launch_bmv<32, OP>(a, s);
-    default: return launch_bmv<64, OP>(a, s);
-  }
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(nbr >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
+    BGB_REQUIRE(C && rowptr, BGB_ERR_ARG, "null output or rowptr");
+    BGB_REQUIRE(nblocks == 0 || (A && B && colind), BGB_ERR_ARG, "null matrix / vector");
+    BGB_REQUIRE(OP != OP_MASKED || mask, BGB_ERR_ARG, "null mask");
+    BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
+    BGB_REQUIRE((long long)nbr * T < INT_MAX, BGB_ERR_SIZE, "NBR*T exceeds the int32 range");
+    auto aligned16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    BGB_REQUIRE(aligned16(A) && aligned16(colind), BGB_ERR_ARG,
+                "bsrVal and bsrColInd must be 16-byte aligned");
+    BGB_REQUIRE(OP == OP_BITS || OP == OP_MASKED || aligned16(C), BGB_ERR_ARG,
+                "count / float outputs must be 16-byte aligned");
+    BGB_REQUIRE((reinterpret_cast<uintptr_t>(C) & 3) == 0, BGB_ERR_ARG, "outputs must be 4-byte aligned");
+    BGB_REQUIRE(T != 64 || !(OP == OP_BITS || OP == OP_MASKED) ||
+                    ((reinterpret_cast<uintptr_t>(C) | reinterpret_cast<uintptr_t>(mask)) & 7) == 0,
+                BGB_ERR_ARG, "T=64 bit outputs and masks must be 8-byte aligned");
+    if (nbr == 0) return BGB_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // a captured call would keep pointers to the per-stream workspace and
+    // the cached partition, which later calls may free or reuse
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    BGB_HIP_TRY(hipStreamIsCapturing(s, &cap));
+    BGB_REQUIRE(cap == hipStreamCaptureStatusNone, BGB_ERR_ARG,
+                "BMV entry points cannot run while their stream is being captured (their carry workspace and "
+                "chunk partition are library-cached and may be freed or reused by later calls)");
+    if (nblocks == 0) {  // every block row is empty
+      if (accumulate && OP != OP_MASKED) return BGB_OK;
+      const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)nbr * word_bytes(T)
+                                                             : (long long)nbr * T * 4;
+      fill_kernel<uint8_t><<<capped_blocks(n, 256), 256, 0, s>>>(static_cast<uint8_t *>(C), n, 0);
+      BGB_HIP_TRY(hipGetLastError());
+      return BGB_OK;
+    }
+    BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr};
+    switch (T) {
+      case 4: return launch_bmv<4, OP>(a, s);
+      case 8: return launch_bmv<8, OP>(a, s);
+      case 16: return launch_bmv<16, OP>(a, s);
+      case 32: return launch_bmv<32, OP>(a, s);
+      default: return launch_bmv<64, OP>(a, s);
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -997,6 +1086,62 @@ int max_row_tiles(hipStream_t s, const int *rowptr, int nbr, int nblocks, int *o
   std::lock_guard<std::mutex> lk(g_rowstat_mu);
   if ((int)g_rowstat.size() >= kRowStatCache) g_rowstat.erase(g_rowstat.begin());
   g_rowstat.push_back(RowStat{dev, rowptr, nbr, nblocks, h});
+  *out = h;
+  return BGB_OK;
+}
+
+// Largest block column of a matrix, per (device, colind address, nblocks):
+// one reduction and a read-back on the first call (the stream is
+// synchronised), kept for the next; -1 while the stream is being captured.
+// It sizes the packed x the LDS-staged kernels copy, so a caller that
+// rewrites colind in place with larger block columns must gather x from
+// memory (bgb_set_x_staging(T, 1)).
+namespace {
+struct ColStat {
+  int dev;
+  const int *colind;
+  int nblocks, max_col;
+};
+std::mutex g_colstat_mu;
+std::vector<ColStat> g_colstat;  // most recently used last
+}  // namespace
+
+int max_block_col(hipStream_t s, const int *colind, int nblocks, int *out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_colstat_mu);
+    for (size_t i = 0; i < g_colstat.size(); ++i) {
+      const ColStat e = g_colstat[i];
+      if (e.dev == dev && e.colind == colind && e.nblocks == nblocks) {
+        g_colstat.erase(g_colstat.begin() + (long)i);
+        g_colstat.push_back(e);
+        *out = e.max_col;
+        return BGB_OK;
+      }
+    }
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  BGB_HIP_TRY(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone || nblocks == 0) {
+    *out = nblocks == 0 ? 0 : -1;
+    return BGB_OK;
+  }
+  // the stream's workspace holds the result: work queued on the stream
+  // before this call is done with it by the time the kernel runs
+  void *ws = nullptr;
+  int st = stream_workspace(s, 256, &ws);
+  if (st) return st;
+  fill_kernel<int><<<1, 64, 0, s>>>(static_cast<int *>(ws), 1, 0);
+  max_col_kernel<<<std::min<unsigned>(capped_blocks(nblocks, 256), kMaxRowBlocks), 256, 0, s>>>(
+      colind, nblocks, static_cast<int *>(ws));
+  BGB_HIP_TRY(hipGetLastError());
+  int h = 0;
+  BGB_HIP_TRY(hipMemcpyAsync(&h, ws, sizeof(int), hipMemcpyDeviceToHost, s));
+  BGB_HIP_TRY(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(g_colstat_mu);
+  if ((int)g_colstat.size() >= kRowStatCache) g_colstat.erase(g_colstat.begin());
+  g_colstat.push_back(ColStat{dev, colind, nblocks, h});
   *out = h;
   return BGB_OK;
 }

@@ -45,6 +45,9 @@ int stream_workspace(hipStream_t s, size_t bytes, void **out);
 // rowptr address, sizes); the kernels validate and repair entries.
 int chunk_partition(hipStream_t s, const int *rowptr, int nbr, int nblocks, int CH, int **out);
 int max_row_tiles(hipStream_t s, const int *rowptr, int nbr, int nblocks, int *out);
+// Largest block column of colind[0..nblocks), cached per (device, colind
+// address, nblocks); -1 when unknown (stream being captured).
+int max_block_col(hipStream_t s, const int *colind, int nblocks, int *out);
 
 // Per-device launch facts, queried once per device: compute units and the
 // LDS bytes a workgroup / a CU may use.

@@ -6,6 +6,9 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
+#include <stdexcept>
 
 #include "../../include/bgb.h"
 
@@ -14,6 +17,24 @@ namespace bgb {
 // Records a message for bgb_last_error() on this host thread and returns status.
 int set_error(int status, const char *fmt, ...);
 void clear_error();
+
+// Runs an entry point's body and turns a C++ exception from host code (a
+// host allocation that fails, a container length beyond its limit) into a
+// status: no exception crosses the C ABI.
+template <class F>
+int host_guard(F &&body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return set_error(BGB_ERR_ALLOC, "host allocation failed");
+  } catch (const std::length_error &e) {
+    return set_error(BGB_ERR_SIZE, "size exceeds a host container limit (%s)", e.what());
+  } catch (const std::exception &e) {
+    return set_error(BGB_ERR_ARG, "host exception: %s", e.what());
+  } catch (...) {
+    return set_error(BGB_ERR_ARG, "unknown host exception");
+  }
+}
 
 inline bool valid_tiledim(int T) { return T == 4 || T == 8 || T == 16 || T == 32 || T == 64; }
 inline int word_bytes(int T) {

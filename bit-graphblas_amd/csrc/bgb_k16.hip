@@ -374,115 +374,121 @@ using namespace bgb;
 extern "C" {
 
 int bgb_k16_pack_rhs(const uint16_t *d_X, int n, void *d_Xb, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(n >= 0 && d_Xb && (n == 0 || d_X), BGB_ERR_ARG, "bad RHS arguments");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const long long nb16 = ceil_div(n, 16);
-  k16_pack_rhs_kernel<<<nb256(nb16 * 256), 256, 0, s>>>(d_X, n, nb16, static_cast<uint16_t *>(d_Xb));
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(n >= 0 && d_Xb && (n == 0 || d_X), BGB_ERR_ARG, "bad RHS arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nb16 = ceil_div(n, 16);
+    k16_pack_rhs_kernel<<<nb256(nb16 * 256), 256, 0, s>>>(d_X, n, nb16, static_cast<uint16_t *>(d_Xb));
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  });
 }
 
 int bgb_bmv_bin_full_full_k16(int T, const void *d_A, const void *d_Xb, float *d_Y, const int *d_rowptr,
                               const int *d_colind, int nblockrows, int nblocks, int accumulate,
                               bgb_stream_t stream) {
-  clear_error();
-  int st = k16_check(T, d_A, d_Xb, d_Y, d_rowptr, d_colind, nblockrows, nblocks);
-  if (st) return st;
-  BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
-  if (nblockrows == 0) return BGB_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (nblocks == 0) {
-    if (!accumulate)
-      k16_fill_kernel<float><<<nb256((long long)nblockrows * T * kK), 256, 0, s>>>(d_Y, (long long)nblockrows * T * kK, 0.f);
+  return host_guard([&]() -> int {
+    clear_error();
+    int st = k16_check(T, d_A, d_Xb, d_Y, d_rowptr, d_colind, nblockrows, nblocks);
+    if (st) return st;
+    BGB_REQUIRE(accumulate == 0 || accumulate == 1, BGB_ERR_ARG, "accumulate must be 0 or 1");
+    if (nblockrows == 0) return BGB_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (nblocks == 0) {
+      if (!accumulate)
+        k16_fill_kernel<float><<<nb256((long long)nblockrows * T * kK), 256, 0, s>>>(d_Y, (long long)nblockrows * T * kK, 0.f);
+      BGB_HIP_TRY(hipGetLastError());
+      return BGB_OK;
+    }
+    const long long nchunks = ceil_div(nblocks, kChunk);
+    const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
+    void *ws = nullptr;
+    if ((st = stream_workspace(s, key_bytes + sizeof(float) * 32 * kK * (size_t)nchunks, &ws))) return st;
+    K16Args a;
+    a.g = K16Graph{d_rowptr, d_colind, d_A, static_cast<const uint2 *>(d_Xb), d_Y, nblockrows, nblocks, accumulate};
+    a.nchunks = (int)nchunks;
+    a.carry_key = static_cast<int *>(ws);
+    a.carry_val = reinterpret_cast<float *>(static_cast<char *>(ws) + key_bytes);
+    const unsigned blocks = (unsigned)ceil_div(nchunks, kWaves);
+    if (T == 16) k16_kernel<16><<<blocks, 64 * kWaves, 0, s>>>(a);
+    else k16_kernel<32><<<blocks, 64 * kWaves, 0, s>>>(a);
+    BGB_HIP_TRY(hipGetLastError());
+    k16_fixup_kernel<<<nb256(ceil_div(nchunks, kFixChunks) * 64), 256, 0, s>>>(a.carry_key, a.carry_val, nchunks, nullptr, nullptr,
+                                                             nullptr, a.g, T);
     BGB_HIP_TRY(hipGetLastError());
     return BGB_OK;
-  }
-  const long long nchunks = ceil_div(nblocks, kChunk);
-  const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
-  void *ws = nullptr;
-  if ((st = stream_workspace(s, key_bytes + sizeof(float) * 32 * kK * (size_t)nchunks, &ws))) return st;
-  K16Args a;
-  a.g = K16Graph{d_rowptr, d_colind, d_A, static_cast<const uint2 *>(d_Xb), d_Y, nblockrows, nblocks, accumulate};
-  a.nchunks = (int)nchunks;
-  a.carry_key = static_cast<int *>(ws);
-  a.carry_val = reinterpret_cast<float *>(static_cast<char *>(ws) + key_bytes);
-  const unsigned blocks = (unsigned)ceil_div(nchunks, kWaves);
-  if (T == 16) k16_kernel<16><<<blocks, 64 * kWaves, 0, s>>>(a);
-  else k16_kernel<32><<<blocks, 64 * kWaves, 0, s>>>(a);
-  BGB_HIP_TRY(hipGetLastError());
-  k16_fixup_kernel<<<nb256(ceil_div(nchunks, kFixChunks) * 64), 256, 0, s>>>(a.carry_key, a.carry_val, nchunks, nullptr, nullptr,
-                                                           nullptr, a.g, T);
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  });
 }
 
 int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_rowptr, const int *d_colind,
                     const void *d_val, const void *d_Xb, float *d_Y, int nworkgroups, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(ngraphs >= 1 && h_graphs && d_rowptr && d_Y, BGB_ERR_ARG, "bad batch arguments");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  std::vector<K16Graph> gs(ngraphs);
-  std::vector<int> td(ngraphs);
-  std::vector<long long> cs(ngraphs + 1, 0);
-  // T = 16 graphs first, then T = 32: one contiguous chunk range per tile dim
-  std::vector<int> order;
-  order.reserve(ngraphs);
-  for (int pass = 0; pass < 2; ++pass)
-    for (int i = 0; i < ngraphs; ++i)
-      if ((h_graphs[i].T == 32) == (pass == 1)) order.push_back(i);
-  int n16 = 0;
-  for (int i = 0; i < ngraphs; ++i) n16 += h_graphs[i].T != 32;
-  for (int slot = 0; slot < ngraphs; ++slot) {
-    const int i = order[slot];
-    const bgb_k16_graph &h = h_graphs[i];
-    BGB_REQUIRE(h.T == 16 || h.T == 32, BGB_ERR_TILEDIM, "graph %d: tile dim %d not in {16, 32}", i, h.T);
-    BGB_REQUIRE(h.n >= 1 && h.nblocks >= 0 && h.val_byte_off % 16 == 0 && h.x_off % 256 == 0, BGB_ERR_ARG,
-                "graph %d: bad sizes or misaligned offsets", i);
-    const int nbr = (int)ceil_div(h.n, h.T);
-    gs[slot] = K16Graph{d_rowptr + h.rowptr_off, d_colind + h.tile_off,
-                        static_cast<const char *>(d_val) + h.val_byte_off,
-                        reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(d_Xb) + h.x_off),
-                        d_Y + h.y_off, nbr, h.nblocks, 0};
-    td[slot] = h.T;
-    cs[slot + 1] = cs[slot] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
-    if (h.nblocks == 0)
-      k16_fill_kernel<float><<<nb256((long long)nbr * h.T * kK), 256, 0, s>>>(d_Y + h.y_off, (long long)nbr * h.T * kK, 0.f);
-  }
-  const long long total = cs[ngraphs];
-  DeviceBuffer dg, dt, dc, q, ck, cv, go;
-  BGB_HIP_TRY(dg.alloc(sizeof(K16Graph) * ngraphs, s));
-  BGB_HIP_TRY(dt.alloc(sizeof(int) * ngraphs, s));
-  BGB_HIP_TRY(dc.alloc(sizeof(long long) * (ngraphs + 1), s));
-  BGB_HIP_TRY(q.alloc(2 * sizeof(unsigned long long), s));  // one queue per tile dim
-  BGB_HIP_TRY(ck.alloc(sizeof(int) * (total + 1), s));
-  BGB_HIP_TRY(cv.alloc(sizeof(float) * 32 * kK * (total + 1), s));
-  BGB_HIP_TRY(go.alloc(sizeof(int) * (total + 1), s));
-  BGB_HIP_TRY(hipMemcpyAsync(dg.ptr, gs.data(), sizeof(K16Graph) * ngraphs, hipMemcpyHostToDevice, s));
-  BGB_HIP_TRY(hipMemcpyAsync(dt.ptr, td.data(), sizeof(int) * ngraphs, hipMemcpyHostToDevice, s));
-  BGB_HIP_TRY(hipMemcpyAsync(dc.ptr, cs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
-  k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 2, 0ull);
-  BGB_HIP_TRY(hipGetLastError());
-  if (total > 0) {
-    const long long split = cs[n16];  // chunks of the T = 16 graphs
-    for (int t = 0; t < 2; ++t) {
-      const long long cb = t == 0 ? 0 : split, ce = t == 0 ? split : total;
-      if (ce <= cb) continue;
-      K16BatchArgs a{dg.as<K16Graph>(), dt.as<int>(), dc.as<long long>(), ngraphs, cb, ce,
-                     q.as<unsigned long long>() + t, ck.as<int>(), cv.as<float>(), go.as<int>()};
-      int wgs = nworkgroups > 0 ? nworkgroups : 2048;
-      if ((long long)wgs * kWaves > ce - cb) wgs = (int)ceil_div(ce - cb, kWaves);
-      if (t == 0) k16_persistent_kernel<16><<<wgs, 64 * kWaves, 0, s>>>(a);
-      else k16_persistent_kernel<32><<<wgs, 64 * kWaves, 0, s>>>(a);
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(ngraphs >= 1 && h_graphs && d_rowptr && d_Y, BGB_ERR_ARG, "bad batch arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<K16Graph> gs(ngraphs);
+    std::vector<int> td(ngraphs);
+    std::vector<long long> cs(ngraphs + 1, 0);
+    // T = 16 graphs first, then T = 32: one contiguous chunk range per tile dim
+    std::vector<int> order;
+    order.reserve(ngraphs);
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < ngraphs; ++i)
+        if ((h_graphs[i].T == 32) == (pass == 1)) order.push_back(i);
+    int n16 = 0;
+    for (int i = 0; i < ngraphs; ++i) n16 += h_graphs[i].T != 32;
+    for (int slot = 0; slot < ngraphs; ++slot) {
+      const int i = order[slot];
+      const bgb_k16_graph &h = h_graphs[i];
+      BGB_REQUIRE(h.T == 16 || h.T == 32, BGB_ERR_TILEDIM, "graph %d: tile dim %d not in {16, 32}", i, h.T);
+      BGB_REQUIRE(h.n >= 1 && h.nblocks >= 0 && h.val_byte_off % 16 == 0 && h.x_off % 256 == 0, BGB_ERR_ARG,
+                  "graph %d: bad sizes or misaligned offsets", i);
+      const int nbr = (int)ceil_div(h.n, h.T);
+      gs[slot] = K16Graph{d_rowptr + h.rowptr_off, d_colind + h.tile_off,
+                          static_cast<const char *>(d_val) + h.val_byte_off,
+                          reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(d_Xb) + h.x_off),
+                          d_Y + h.y_off, nbr, h.nblocks, 0};
+      td[slot] = h.T;
+      cs[slot + 1] = cs[slot] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
+      if (h.nblocks == 0)
+        k16_fill_kernel<float><<<nb256((long long)nbr * h.T * kK), 256, 0, s>>>(d_Y + h.y_off, (long long)nbr * h.T * kK, 0.f);
+    }
+    const long long total = cs[ngraphs];
+    DeviceBuffer dg, dt, dc, q, ck, cv, go;
+    BGB_HIP_TRY(dg.alloc(sizeof(K16Graph) * ngraphs, s));
+    BGB_HIP_TRY(dt.alloc(sizeof(int) * ngraphs, s));
+    BGB_HIP_TRY(dc.alloc(sizeof(long long) * (ngraphs + 1), s));
+    BGB_HIP_TRY(q.alloc(2 * sizeof(unsigned long long), s));  // one queue per tile dim
+    BGB_HIP_TRY(ck.alloc(sizeof(int) * (total + 1), s));
+    BGB_HIP_TRY(cv.alloc(sizeof(float) * 32 * kK * (total + 1), s));
+    BGB_HIP_TRY(go.alloc(sizeof(int) * (total + 1), s));
+    BGB_HIP_TRY(hipMemcpyAsync(dg.ptr, gs.data(), sizeof(K16Graph) * ngraphs, hipMemcpyHostToDevice, s));
+    BGB_HIP_TRY(hipMemcpyAsync(dt.ptr, td.data(), sizeof(int) * ngraphs, hipMemcpyHostToDevice, s));
+    BGB_HIP_TRY(hipMemcpyAsync(dc.ptr, cs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
+    k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 2, 0ull);
+    BGB_HIP_TRY(hipGetLastError());
+    if (total > 0) {
+      const long long split = cs[n16];  // chunks of the T = 16 graphs
+      for (int t = 0; t < 2; ++t) {
+        const long long cb = t == 0 ? 0 : split, ce = t == 0 ? split : total;
+        if (ce <= cb) continue;
+        K16BatchArgs a{dg.as<K16Graph>(), dt.as<int>(), dc.as<long long>(), ngraphs, cb, ce,
+                       q.as<unsigned long long>() + t, ck.as<int>(), cv.as<float>(), go.as<int>()};
+        int wgs = nworkgroups > 0 ? nworkgroups : 2048;
+        if ((long long)wgs * kWaves > ce - cb) wgs = (int)ceil_div(ce - cb, kWaves);
+        if (t == 0) k16_persistent_kernel<16><<<wgs, 64 * kWaves, 0, s>>>(a);
+        else k16_persistent_kernel<32><<<wgs, 64 * kWaves, 0, s>>>(a);
+        BGB_HIP_TRY(hipGetLastError());
+      }
+      K16Graph none{};
+      k16_fixup_kernel<<<nb256(ceil_div(total, kFixChunks) * 64), 256, 0, s>>>(ck.as<int>(), cv.as<float>(), total, dg.as<K16Graph>(),
+                                                             go.as<int>(), dt.as<int>(), none, 0);
       BGB_HIP_TRY(hipGetLastError());
     }
-    K16Graph none{};
-    k16_fixup_kernel<<<nb256(ceil_div(total, kFixChunks) * 64), 256, 0, s>>>(ck.as<int>(), cv.as<float>(), total, dg.as<K16Graph>(),
-                                                           go.as<int>(), dt.as<int>(), none, 0);
-    BGB_HIP_TRY(hipGetLastError());
-  }
-  BGB_HIP_TRY(hipStreamSynchronize(s));
-  return BGB_OK;
+    BGB_HIP_TRY(hipStreamSynchronize(s));
+    return BGB_OK;
+  });
 }
 
 }  // extern "C"

@@ -71,76 +71,81 @@ using namespace bgb;
 extern "C" {
 
 int bgb_read_mtx(const char *path, int directed, bgb_csr **out) {
-  clear_error();
-  BGB_REQUIRE(path && out, BGB_ERR_ARG, "null path or output");
-  BGB_REQUIRE(directed >= 0 && directed <= 2, BGB_ERR_ARG, "directed must be 0, 1 or 2 (got %d)", directed);
-  FILE *f = std::fopen(path, "rb");
-  BGB_REQUIRE(f, BGB_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
-  std::string buf;
-  {
-    char chunk[1 << 16];
-    size_t got;
-    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
-    std::fclose(f);
-  }
-  size_t pos = 0;
-  const char *b = nullptr, *e = nullptr;
-  // banner
-  BGB_REQUIRE(next_line(buf, &pos, &b, &e), BGB_ERR_FORMAT, "%s: empty file", path);
-  std::vector<std::string> tok;
-  {
-    std::string line(b, e), t;
-    for (char c : line) {
-      if (std::isspace((unsigned char)c)) {
-        if (!t.empty()) tok.push_back(lower(t)), t.clear();
-      } else {
-        t += c;
-      }
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(path && out, BGB_ERR_ARG, "null path or output");
+    BGB_REQUIRE(directed >= 0 && directed <= 2, BGB_ERR_ARG, "directed must be 0, 1 or 2 (got %d)", directed);
+    FILE *f = std::fopen(path, "rb");
+    BGB_REQUIRE(f, BGB_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
+    std::string buf;
+    {
+      char chunk[1 << 16];
+      size_t got;
+      while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
+      std::fclose(f);
     }
-    if (!t.empty()) tok.push_back(lower(t));
-  }
-  BGB_REQUIRE(tok.size() == 5 && tok[0] == "%%matrixmarket" && tok[1] == "matrix", BGB_ERR_FORMAT,
-              "%s: not a MatrixMarket matrix banner", path);
-  BGB_REQUIRE(tok[2] == "coordinate", BGB_ERR_FORMAT, "%s: only the coordinate format is read (got %s)", path,
-              tok[2].c_str());
-  const std::string &field = tok[3], &sym = tok[4];
-  BGB_REQUIRE(field == "real" || field == "integer" || field == "pattern", BGB_ERR_FORMAT,
-              "%s: field %s not supported (real, integer or pattern)", path, field.c_str());
-  BGB_REQUIRE(sym == "general" || sym == "symmetric" || sym == "skew-symmetric" || sym == "hermitian",
-              BGB_ERR_FORMAT, "%s: unknown symmetry %s", path, sym.c_str());
-  // size line after comments and blank lines
-  long long M = -1, N = -1, NNZ = -1;
-  for (;;) {
-    BGB_REQUIRE(next_line(buf, &pos, &b, &e), BGB_ERR_FORMAT, "%s: no size line", path);
-    if (b < e && *b == '%') continue;
-    if (blank(b, e)) continue;
-    const char *p = b;
-    BGB_REQUIRE(parse_ll(&p, e, &M) && parse_ll(&p, e, &N) && parse_ll(&p, e, &NNZ), BGB_ERR_FORMAT,
-                "%s: bad size line", path);
-    break;
-  }
-  BGB_REQUIRE(M >= 0 && N >= 0 && NNZ >= 0 && M <= INT32_MAX && N <= INT32_MAX, BGB_ERR_FORMAT,
-              "%s: sizes %lld x %lld, %lld entries out of range", path, M, N, NNZ);
-  std::vector<int> src, dst;
-  src.reserve((size_t)NNZ);
-  dst.reserve((size_t)NNZ);
-  long long line_no = 0;
-  while ((long long)src.size() < NNZ && next_line(buf, &pos, &b, &e)) {
-    ++line_no;
-    if (blank(b, e) || *b == '%') continue;
-    const char *p = b;
-    long long i = 0, j = 0;
-    BGB_REQUIRE(parse_ll(&p, e, &i) && parse_ll(&p, e, &j), BGB_ERR_FORMAT, "%s: bad entry line %lld", path,
-                line_no);
-    BGB_REQUIRE(i >= 1 && i <= M && j >= 1 && j <= N, BGB_ERR_FORMAT,
-                "%s: entry (%lld, %lld) outside the %lld x %lld matrix", path, i, j, M, N);
-    src.push_back((int)(i - 1));
-    dst.push_back((int)(j - 1));
-  }
-  BGB_REQUIRE((long long)src.size() == NNZ, BGB_ERR_FORMAT, "%s: %lld entries announced, %zu found", path, NNZ,
-              src.size());
-  const bool undirected = directed == 2 || (directed == 0 && sym == "symmetric");
-  return bgb_csr_from_coo((int)M, (int)N, NNZ, src.data(), dst.data(), undirected ? 1 : 0, 1, out);
+    size_t pos = 0;
+    const char *b = nullptr, *e = nullptr;
+    // banner
+    BGB_REQUIRE(next_line(buf, &pos, &b, &e), BGB_ERR_FORMAT, "%s: empty file", path);
+    std::vector<std::string> tok;
+    {
+      std::string line(b, e), t;
+      for (char c : line) {
+        if (std::isspace((unsigned char)c)) {
+          if (!t.empty()) tok.push_back(lower(t)), t.clear();
+        } else {
+          t += c;
+        }
+      }
+      if (!t.empty()) tok.push_back(lower(t));
+    }
+    BGB_REQUIRE(tok.size() == 5 && tok[0] == "%%matrixmarket" && tok[1] == "matrix", BGB_ERR_FORMAT,
+                "%s: not a MatrixMarket matrix banner", path);
+    BGB_REQUIRE(tok[2] == "coordinate", BGB_ERR_FORMAT, "%s: only the coordinate format is read (got %s)", path,
+                tok[2].c_str());
+    const std::string &field = tok[3], &sym = tok[4];
+    BGB_REQUIRE(field == "real" || field == "integer" || field == "pattern", BGB_ERR_FORMAT,
+                "%s: field %s not supported (real, integer or pattern)", path, field.c_str());
+    BGB_REQUIRE(sym == "general" || sym == "symmetric" || sym == "skew-symmetric" || sym == "hermitian",
+                BGB_ERR_FORMAT, "%s: unknown symmetry %s", path, sym.c_str());
+    // size line after comments and blank lines
+    long long M = -1, N = -1, NNZ = -1;
+    for (;;) {
+      BGB_REQUIRE(next_line(buf, &pos, &b, &e), BGB_ERR_FORMAT, "%s: no size line", path);
+      if (b < e && *b == '%') continue;
+      if (blank(b, e)) continue;
+      const char *p = b;
+      BGB_REQUIRE(parse_ll(&p, e, &M) && parse_ll(&p, e, &N) && parse_ll(&p, e, &NNZ), BGB_ERR_FORMAT,
+                  "%s: bad size line", path);
+      break;
+    }
+    BGB_REQUIRE(M >= 0 && N >= 0 && NNZ >= 0 && M <= INT32_MAX && N <= INT32_MAX, BGB_ERR_FORMAT,
+                "%s: sizes %lld x %lld, %lld entries out of range", path, M, N, NNZ);
+    // an entry line takes at least 3 bytes ("i j"), so the file bounds the
+    // count it can hold whatever its size line announces
+    const long long fits = (long long)((buf.size() - pos) / 3 + 1);
+    std::vector<int> src, dst;
+    src.reserve((size_t)(NNZ < fits ? NNZ : fits));
+    dst.reserve((size_t)(NNZ < fits ? NNZ : fits));
+    long long line_no = 0;
+    while ((long long)src.size() < NNZ && next_line(buf, &pos, &b, &e)) {
+      ++line_no;
+      if (blank(b, e) || *b == '%') continue;
+      const char *p = b;
+      long long i = 0, j = 0;
+      BGB_REQUIRE(parse_ll(&p, e, &i) && parse_ll(&p, e, &j), BGB_ERR_FORMAT, "%s: bad entry line %lld", path,
+                  line_no);
+      BGB_REQUIRE(i >= 1 && i <= M && j >= 1 && j <= N, BGB_ERR_FORMAT,
+                  "%s: entry (%lld, %lld) outside the %lld x %lld matrix", path, i, j, M, N);
+      src.push_back((int)(i - 1));
+      dst.push_back((int)(j - 1));
+    }
+    BGB_REQUIRE((long long)src.size() == NNZ, BGB_ERR_FORMAT, "%s: %lld entries announced, %zu found", path, NNZ,
+                src.size());
+    const bool undirected = directed == 2 || (directed == 0 && sym == "symmetric");
+    return bgb_csr_from_coo((int)M, (int)N, NNZ, src.data(), dst.data(), undirected ? 1 : 0, 1, out);
+  });
 }
 
 }  // extern "C"

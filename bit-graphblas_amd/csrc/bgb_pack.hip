@@ -268,89 +268,95 @@ extern "C" {
 
 int bgb_b2sr_nnzb(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
                   const int *d_csrColInd, int *d_bsrRowPtr, int *h_nblocks, bgb_stream_t stream) {
-  clear_error();
-  int st = check_common(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd);
-  if (st) return st;
-  BGB_REQUIRE(d_bsrRowPtr && h_nblocks, BGB_ERR_ARG, "null output");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((st = validate_csr(d_csrRowPtr, d_csrColInd, nrows, ncols, nnz, s))) return st;
-  DeviceBuffer keys, flags, excl;
-  KeyLayout kl;
-  if ((st = sorted_tiles(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd, s, keys, flags, excl, kl)))
-    return st;
-  const int nbr = (int)ceil_div(nrows, T);
-  block_rowptr_kernel<<<grid_for((long long)nbr + 1), kBlock, 0, s>>>(d_csrRowPtr, nrows, T, nbr,
-                                                                     excl.as<long long>(), d_bsrRowPtr);
-  BGB_HIP_TRY(hipGetLastError());
-  long long total = 0;
-  BGB_HIP_TRY(hipMemcpyAsync(&total, excl.as<long long>() + nnz, sizeof(long long),
-                             hipMemcpyDeviceToHost, s));
-  BGB_HIP_TRY(hipStreamSynchronize(s));
-  *h_nblocks = (int)total;
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    int st = check_common(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd);
+    if (st) return st;
+    BGB_REQUIRE(d_bsrRowPtr && h_nblocks, BGB_ERR_ARG, "null output");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = validate_csr(d_csrRowPtr, d_csrColInd, nrows, ncols, nnz, s))) return st;
+    DeviceBuffer keys, flags, excl;
+    KeyLayout kl;
+    if ((st = sorted_tiles(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd, s, keys, flags, excl, kl)))
+      return st;
+    const int nbr = (int)ceil_div(nrows, T);
+    block_rowptr_kernel<<<grid_for((long long)nbr + 1), kBlock, 0, s>>>(d_csrRowPtr, nrows, T, nbr,
+                                                                       excl.as<long long>(), d_bsrRowPtr);
+    BGB_HIP_TRY(hipGetLastError());
+    long long total = 0;
+    BGB_HIP_TRY(hipMemcpyAsync(&total, excl.as<long long>() + nnz, sizeof(long long),
+                               hipMemcpyDeviceToHost, s));
+    BGB_HIP_TRY(hipStreamSynchronize(s));
+    *h_nblocks = (int)total;
+    return BGB_OK;
+  });
 }
 
 int bgb_csr2b2sr(int T, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
                  const int *d_csrColInd, const int *d_bsrRowPtr, int nblocks, int *d_bsrColInd,
                  void *d_bsrVal, bgb_stream_t stream) {
-  clear_error();
-  int st = check_common(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd);
-  if (st) return st;
-  BGB_REQUIRE(nblocks >= 0 && d_bsrRowPtr, BGB_ERR_ARG, "bad block row pointer / nblocks");
-  BGB_REQUIRE(nblocks == 0 || (d_bsrColInd && d_bsrVal), BGB_ERR_ARG, "null output");
-  BGB_REQUIRE(nblocks <= nnz, BGB_ERR_ARG, "nblocks %d exceeds nnz %d", nblocks, nnz);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((st = validate_csr(d_csrRowPtr, d_csrColInd, nrows, ncols, nnz, s))) return st;
-  if (nblocks == 0 && nnz == 0) return BGB_OK;
-  DeviceBuffer keys, flags, excl, heads;
-  KeyLayout kl;
-  if ((st = sorted_tiles(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd, s, keys, flags, excl, kl)))
-    return st;
-  long long total = 0;
-  BGB_HIP_TRY(hipMemcpyAsync(&total, excl.as<long long>() + nnz, sizeof(long long),
-                             hipMemcpyDeviceToHost, s));
-  BGB_HIP_TRY(hipStreamSynchronize(s));
-  BGB_REQUIRE(total == nblocks, BGB_ERR_ARG, "nblocks %d does not match the matrix (%lld tiles)",
-              nblocks, total);
-  BGB_HIP_TRY(heads.alloc(sizeof(int) * ((size_t)nblocks + 1), s));
-  tile_heads_kernel<<<grid_for((long long)nnz + 1), kBlock, 0, s>>>(
-      keys.as<uint64_t>(), flags.as<uint8_t>(), excl.as<long long>(), nnz, kl, d_bsrColInd,
-      heads.as<int>());
-  BGB_HIP_TRY(hipGetLastError());
-  const unsigned g = grid_for(nblocks);
-  switch (T) {
-    case 4: tile_fill_kernel<4><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
-    case 8: tile_fill_kernel<8><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
-    case 16: tile_fill_kernel<16><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
-    case 32: tile_fill_kernel<32><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
-    default:
-      tile_fill64_kernel<<<capped_blocks(64LL * nblocks, kBlock), kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(),
-                                                                     64LL * nblocks, static_cast<uint64_t *>(d_bsrVal));
-      break;
-  }
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    int st = check_common(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd);
+    if (st) return st;
+    BGB_REQUIRE(nblocks >= 0 && d_bsrRowPtr, BGB_ERR_ARG, "bad block row pointer / nblocks");
+    BGB_REQUIRE(nblocks == 0 || (d_bsrColInd && d_bsrVal), BGB_ERR_ARG, "null output");
+    BGB_REQUIRE(nblocks <= nnz, BGB_ERR_ARG, "nblocks %d exceeds nnz %d", nblocks, nnz);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = validate_csr(d_csrRowPtr, d_csrColInd, nrows, ncols, nnz, s))) return st;
+    if (nblocks == 0 && nnz == 0) return BGB_OK;
+    DeviceBuffer keys, flags, excl, heads;
+    KeyLayout kl;
+    if ((st = sorted_tiles(T, nrows, ncols, nnz, d_csrRowPtr, d_csrColInd, s, keys, flags, excl, kl)))
+      return st;
+    long long total = 0;
+    BGB_HIP_TRY(hipMemcpyAsync(&total, excl.as<long long>() + nnz, sizeof(long long),
+                               hipMemcpyDeviceToHost, s));
+    BGB_HIP_TRY(hipStreamSynchronize(s));
+    BGB_REQUIRE(total == nblocks, BGB_ERR_ARG, "nblocks %d does not match the matrix (%lld tiles)",
+                nblocks, total);
+    BGB_HIP_TRY(heads.alloc(sizeof(int) * ((size_t)nblocks + 1), s));
+    tile_heads_kernel<<<grid_for((long long)nnz + 1), kBlock, 0, s>>>(
+        keys.as<uint64_t>(), flags.as<uint8_t>(), excl.as<long long>(), nnz, kl, d_bsrColInd,
+        heads.as<int>());
+    BGB_HIP_TRY(hipGetLastError());
+    const unsigned g = grid_for(nblocks);
+    switch (T) {
+      case 4: tile_fill_kernel<4><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+      case 8: tile_fill_kernel<8><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+      case 16: tile_fill_kernel<16><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+      case 32: tile_fill_kernel<32><<<g, kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(), nblocks, d_bsrVal); break;
+      default:
+        tile_fill64_kernel<<<capped_blocks(64LL * nblocks, kBlock), kBlock, 0, s>>>(keys.as<uint64_t>(), heads.as<int>(),
+                                                                       64LL * nblocks, static_cast<uint64_t *>(d_bsrVal));
+        break;
+    }
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  });
 }
 
 int bgb_pack_vec(int T, const float *d_x, int n, void *d_xb, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(n >= 0 && d_xb && (n == 0 || d_x), BGB_ERR_ARG, "bad vector arguments");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const long long nbr = ceil_div(n, T);
-  const long long nwords = ceil_div(nbr, T) * T;
-  if (nwords == 0) return BGB_OK;
-  const long long threads = ceil_div(nwords * T, 64) * 64;
-  const unsigned g = grid_for(threads);
-  switch (T) {
-    case 4: pack_vec_kernel<4><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-    case 8: pack_vec_kernel<8><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-    case 16: pack_vec_kernel<16><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-    case 32: pack_vec_kernel<32><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-    default: pack_vec_kernel<64><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
-  }
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(n >= 0 && d_xb && (n == 0 || d_x), BGB_ERR_ARG, "bad vector arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nbr = ceil_div(n, T);
+    const long long nwords = ceil_div(nbr, T) * T;
+    if (nwords == 0) return BGB_OK;
+    const long long threads = ceil_div(nwords * T, 64) * 64;
+    const unsigned g = grid_for(threads);
+    switch (T) {
+      case 4: pack_vec_kernel<4><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+      case 8: pack_vec_kernel<8><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+      case 16: pack_vec_kernel<16><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+      case 32: pack_vec_kernel<32><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+      default: pack_vec_kernel<64><<<g, kBlock, 0, s>>>(d_x, n, nwords, d_xb); break;
+    }
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  });
 }
 
 }  // extern "C"

@@ -339,16 +339,18 @@ using namespace bgb;
 extern "C" int bgb_pagerank(int T, const void *d_A, const int *d_rowptr, const int *d_colind, int nblockrows,
                             int nblocks, int n, float damping, int iters, int mode, float *d_pr,
                             double *h_residuals, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(d_rowptr && d_pr && (nblocks == 0 || (d_A && d_colind)), BGB_ERR_ARG, "null argument");
-  BGB_REQUIRE(n >= 1 && nblockrows == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");
-  BGB_REQUIRE(iters >= 0 && damping >= 0.0f && damping <= 1.0f, BGB_ERR_ARG, "bad iters / damping");
-  BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "mode must be 0 (fp32), 1 (bf16 x) or 2 (fp64 sums)");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  switch (mode) {
-    case 0: return pagerank_impl<float, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
-    case 1: return pagerank_impl<float, true>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
-    default: return pagerank_impl<double, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
-  }
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(d_rowptr && d_pr && (nblocks == 0 || (d_A && d_colind)), BGB_ERR_ARG, "null argument");
+    BGB_REQUIRE(n >= 1 && nblockrows == (int)ceil_div(n, T), BGB_ERR_ARG, "nblockrows must be ceil(n/T)");
+    BGB_REQUIRE(iters >= 0 && damping >= 0.0f && damping <= 1.0f, BGB_ERR_ARG, "bad iters / damping");
+    BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "mode must be 0 (fp32), 1 (bf16 x) or 2 (fp64 sums)");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (mode) {
+      case 0: return pagerank_impl<float, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+      case 1: return pagerank_impl<float, true>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+      default: return pagerank_impl<double, false>(T, d_A, d_rowptr, d_colind, nblockrows, nblocks, n, damping, iters, d_pr, h_residuals, s);
+    }
+  });
 }

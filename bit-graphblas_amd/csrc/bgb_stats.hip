@@ -72,37 +72,41 @@ extern "C" {
 
 int bgb_tile_row_stats(int T, const void *d_A, const int *d_rowptr, int nblockrows, int *d_tile_per_row,
                        int *d_nnz_tile_per_row, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(nblockrows >= 0 && d_rowptr && (nblockrows == 0 || (d_tile_per_row && d_nnz_tile_per_row)),
-              BGB_ERR_ARG, "bad arguments");
-  if (nblockrows == 0) return BGB_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const unsigned grid = capped_blocks((long long)nblockrows * 64, kBlock);
-  switch (T) {
-    case 4: tile_row_stats_kernel<4><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
-    case 8: tile_row_stats_kernel<8><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
-    case 16: tile_row_stats_kernel<16><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
-    case 32: tile_row_stats_kernel<32><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
-    default: tile_row_stats_kernel<64><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
-  }
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(nblockrows >= 0 && d_rowptr && (nblockrows == 0 || (d_tile_per_row && d_nnz_tile_per_row)),
+                BGB_ERR_ARG, "bad arguments");
+    if (nblockrows == 0) return BGB_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned grid = capped_blocks((long long)nblockrows * 64, kBlock);
+    switch (T) {
+      case 4: tile_row_stats_kernel<4><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
+      case 8: tile_row_stats_kernel<8><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
+      case 16: tile_row_stats_kernel<16><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
+      case 32: tile_row_stats_kernel<32><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
+      default: tile_row_stats_kernel<64><<<grid, kBlock, 0, s>>>(d_A, d_rowptr, nblockrows, d_tile_per_row, d_nnz_tile_per_row); break;
+    }
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  });
 }
 
 int bgb_estimate_tiles(int T, int nrows, const int *d_rowptr, const int *d_colind, long long *d_out,
                        bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(nrows >= 0 && d_rowptr && d_out, BGB_ERR_ARG, "bad arguments");  // colind unread when every row is empty
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  zero_i64_kernel<<<1, 64, 0, s>>>(d_out);
-  BGB_HIP_TRY(hipGetLastError());
-  if (nrows == 0) return BGB_OK;
-  estimate_tiles_kernel<<<capped_blocks((long long)nrows * 64, kBlock), kBlock, 0, s>>>(
-      d_rowptr, d_colind, nrows, ilog2_tiledim(T), reinterpret_cast<unsigned long long *>(d_out));
-  BGB_HIP_TRY(hipGetLastError());
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(nrows >= 0 && d_rowptr && d_out, BGB_ERR_ARG, "bad arguments");  // colind unread when every row is empty
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    zero_i64_kernel<<<1, 64, 0, s>>>(d_out);
+    BGB_HIP_TRY(hipGetLastError());
+    if (nrows == 0) return BGB_OK;
+    estimate_tiles_kernel<<<capped_blocks((long long)nrows * 64, kBlock), kBlock, 0, s>>>(
+        d_rowptr, d_colind, nrows, ilog2_tiledim(T), reinterpret_cast<unsigned long long *>(d_out));
+    BGB_HIP_TRY(hipGetLastError());
+    return BGB_OK;
+  });
 }
 
 }  // extern "C"

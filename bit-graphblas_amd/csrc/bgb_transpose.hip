@@ -205,51 +205,55 @@ extern "C" {
 
 int bgb_csr2csc(int m, int n, int nnz, const float *d_csrVal, const int *d_csrRowPtr, const int *d_csrColInd,
                 float *d_cscVal, int *d_cscRowInd, int *d_cscColPtr, bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(m >= 0 && n >= 0 && nnz >= 0, BGB_ERR_ARG, "negative size");
-  BGB_REQUIRE(d_csrRowPtr && d_cscColPtr && (nnz == 0 || (d_csrColInd && d_cscRowInd)), BGB_ERR_ARG,
-              "null CSR / CSC array");
-  BGB_REQUIRE((d_csrVal == nullptr) == (d_cscVal == nullptr), BGB_ERR_ARG,
-              "values: pass both csrVal and cscVal, or neither (structure only)");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = check_structure(d_csrRowPtr, d_csrColInd, m, n, nnz, s, "CSR");
-  if (st) return st;
-  DeviceBuffer keys, row_of;
-  int pb = 0;
-  if ((st = sorted_entries(d_csrRowPtr, d_csrColInd, m, n, nnz, s, keys, row_of, d_cscColPtr, &pb))) return st;
-  if (nnz > 0) {
-    csc_scatter_kernel<<<capped_blocks(nnz, kBlock), kBlock, 0, s>>>(keys.as<uint64_t>(), nnz, pb, row_of.as<int>(),
-                                                                     d_csrVal, d_cscRowInd, d_cscVal);
-    BGB_HIP_TRY(hipGetLastError());
-  }
-  return BGB_OK;
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(m >= 0 && n >= 0 && nnz >= 0, BGB_ERR_ARG, "negative size");
+    BGB_REQUIRE(d_csrRowPtr && d_cscColPtr && (nnz == 0 || (d_csrColInd && d_cscRowInd)), BGB_ERR_ARG,
+                "null CSR / CSC array");
+    BGB_REQUIRE((d_csrVal == nullptr) == (d_cscVal == nullptr), BGB_ERR_ARG,
+                "values: pass both csrVal and cscVal, or neither (structure only)");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int st = check_structure(d_csrRowPtr, d_csrColInd, m, n, nnz, s, "CSR");
+    if (st) return st;
+    DeviceBuffer keys, row_of;
+    int pb = 0;
+    if ((st = sorted_entries(d_csrRowPtr, d_csrColInd, m, n, nnz, s, keys, row_of, d_cscColPtr, &pb))) return st;
+    if (nnz > 0) {
+      csc_scatter_kernel<<<capped_blocks(nnz, kBlock), kBlock, 0, s>>>(keys.as<uint64_t>(), nnz, pb, row_of.as<int>(),
+                                                                       d_csrVal, d_cscRowInd, d_cscVal);
+      BGB_HIP_TRY(hipGetLastError());
+    }
+    return BGB_OK;
+  });
 }
 
 int bgb_b2sr_transpose(int T, int nblockrows, int nblockcols, int nblocks, const int *d_rowptr,
                        const int *d_colind, const void *d_val, int *d_t_rowptr, int *d_t_colind, void *d_t_val,
                        bgb_stream_t stream) {
-  clear_error();
-  BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
-  BGB_REQUIRE(nblockrows >= 0 && nblockcols >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
-  BGB_REQUIRE(d_rowptr && d_t_rowptr && (nblocks == 0 || (d_colind && d_val && d_t_colind && d_t_val)),
-              BGB_ERR_ARG, "null B2SR array");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = check_structure(d_rowptr, d_colind, nblockrows, nblockcols, nblocks, s, "B2SR");
-  if (st) return st;
-  DeviceBuffer keys, row_of;
-  int pb = 0;
-  if ((st = sorted_entries(d_rowptr, d_colind, nblockrows, nblockcols, nblocks, s, keys, row_of, d_t_rowptr, &pb)))
-    return st;
-  if (nblocks == 0) return BGB_OK;
-  const uint64_t *k = keys.as<uint64_t>();
-  const int *ro = row_of.as<int>();
-  switch (T) {
-    case 4: return launch_tile_transpose<4>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
-    case 8: return launch_tile_transpose<8>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
-    case 16: return launch_tile_transpose<16>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
-    case 32: return launch_tile_transpose<32>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
-    default: return launch_tile_transpose<64>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
-  }
+  return host_guard([&]() -> int {
+    clear_error();
+    BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
+    BGB_REQUIRE(nblockrows >= 0 && nblockcols >= 0 && nblocks >= 0, BGB_ERR_ARG, "negative size");
+    BGB_REQUIRE(d_rowptr && d_t_rowptr && (nblocks == 0 || (d_colind && d_val && d_t_colind && d_t_val)),
+                BGB_ERR_ARG, "null B2SR array");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int st = check_structure(d_rowptr, d_colind, nblockrows, nblockcols, nblocks, s, "B2SR");
+    if (st) return st;
+    DeviceBuffer keys, row_of;
+    int pb = 0;
+    if ((st = sorted_entries(d_rowptr, d_colind, nblockrows, nblockcols, nblocks, s, keys, row_of, d_t_rowptr, &pb)))
+      return st;
+    if (nblocks == 0) return BGB_OK;
+    const uint64_t *k = keys.as<uint64_t>();
+    const int *ro = row_of.as<int>();
+    switch (T) {
+      case 4: return launch_tile_transpose<4>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
+      case 8: return launch_tile_transpose<8>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
+      case 16: return launch_tile_transpose<16>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
+      case 32: return launch_tile_transpose<32>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
+      default: return launch_tile_transpose<64>(k, nblocks, pb, ro, d_val, d_t_colind, d_t_val, s);
+    }
+  });
 }
 
 }  // extern "C"

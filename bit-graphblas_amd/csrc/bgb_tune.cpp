@@ -71,70 +71,76 @@ extern "C" {
 
 int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colind, long long *h_nblocks4,
                    long long *h_nonempty4) {
-  bgb::clear_error();
-  BGB_REQUIRE(nrows >= 0 && ncols >= 0 && h_rowptr && h_nblocks4, BGB_ERR_ARG, "bad arguments");
-  BGB_REQUIRE(h_rowptr[nrows] == 0 || h_colind, BGB_ERR_ARG, "null colind");
-  for (int t = 0; t < 4; ++t) {
-    const int T = kTs[t];
-    const long long nbr = (nrows + T - 1) / T;
-    long long tiles = 0, nonempty = 0;
-#pragma omp parallel reduction(+ : tiles, nonempty)
-    {
-      std::vector<int> buf;
-#pragma omp for schedule(dynamic, 256)
-      for (long long I = 0; I < nbr; ++I) {
-        const int r0 = (int)(I * T), r1 = (int)std::min<long long>((I + 1) * T, nrows);
-        buf.clear();
-        for (int k = h_rowptr[r0]; k < h_rowptr[r1]; ++k) buf.push_back(h_colind[k] / T);
-        if (buf.empty()) continue;
-        std::sort(buf.begin(), buf.end());
-        tiles += std::unique(buf.begin(), buf.end()) - buf.begin();
-        ++nonempty;
+  return bgb::host_guard([&]() -> int {
+    bgb::clear_error();
+    BGB_REQUIRE(nrows >= 0 && ncols >= 0 && h_rowptr && h_nblocks4, BGB_ERR_ARG, "bad arguments");
+    BGB_REQUIRE(h_rowptr[nrows] == 0 || h_colind, BGB_ERR_ARG, "null colind");
+    for (int t = 0; t < 4; ++t) {
+      const int T = kTs[t];
+      const long long nbr = (nrows + T - 1) / T;
+      long long tiles = 0, nonempty = 0;
+  #pragma omp parallel reduction(+ : tiles, nonempty)
+      {
+        std::vector<int> buf;
+  #pragma omp for schedule(dynamic, 256)
+        for (long long I = 0; I < nbr; ++I) {
+          const int r0 = (int)(I * T), r1 = (int)std::min<long long>((I + 1) * T, nrows);
+          buf.clear();
+          for (int k = h_rowptr[r0]; k < h_rowptr[r1]; ++k) buf.push_back(h_colind[k] / T);
+          if (buf.empty()) continue;
+          std::sort(buf.begin(), buf.end());
+          tiles += std::unique(buf.begin(), buf.end()) - buf.begin();
+          ++nonempty;
+        }
       }
+      h_nblocks4[t] = tiles;
+      if (h_nonempty4) h_nonempty4[t] = nonempty;
     }
-    h_nblocks4[t] = tiles;
-    if (h_nonempty4) h_nonempty4[t] = nonempty;
-  }
-  return BGB_OK;
+    return BGB_OK;
+  });
 }
 
 int bgb_autotune(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
                  int *run_length_out, int *x_staging_out, double *cost4_us) {
-  int T = 0;
-  const int st = bgb_autotune_tiledim(op, nrows, h_nblocks4, allowed_mask, &T, cost4_us);
-  if (st) return st;
-  if (T_out) *T_out = T;
-  // run length 16 never beat 8 on MI355X (T = 4, 8 slower; 16, 32 equal)
-  if (run_length_out) *run_length_out = 8;
-  if (x_staging_out) *x_staging_out = (op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX) ? 2 : 1;
-  return BGB_OK;
+  return bgb::host_guard([&]() -> int {
+    int T = 0;
+    const int st = bgb_autotune_tiledim(op, nrows, h_nblocks4, allowed_mask, &T, cost4_us);
+    if (st) return st;
+    if (T_out) *T_out = T;
+    // run length 16 never beat 8 on MI355X (T = 4, 8 slower; 16, 32 equal)
+    if (run_length_out) *run_length_out = 8;
+    if (x_staging_out) *x_staging_out = (op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX) ? 2 : 1;
+    return BGB_OK;
+  });
 }
 
 int bgb_autotune_tiledim(int op, int nrows, const long long *h_nblocks4, int allowed_mask, int *T_out,
                          double *cost4_us) {
-  bgb::clear_error();
-  BGB_REQUIRE(op >= 0 && op <= 3 && nrows >= 0 && h_nblocks4 && T_out, BGB_ERR_ARG, "bad arguments");
-  if (op == 2) allowed_mask &= 0xC;  // the MFMA path takes T = 16, 32
-  BGB_REQUIRE(allowed_mask & 0xF, BGB_ERR_ARG, "no tile dim allowed");
-  const Model &m = kModels[op];
-  double best = 1e300;
-  for (int t = 0; t < 4; ++t) {
-    const int T = kTs[t];
-    const double nbr = (double)((nrows + T - 1) / T);
-    const double mem_us = op_bytes(op, T, nrows, h_nblocks4[t]) / (m.bw_gbs * 1e3);
-    const bool lds = op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX;
-    const double tile_ns = lds ? kTileNsLds[t] : m.tile_ns[t];
-    const double issue_us = ((double)h_nblocks4[t] * tile_ns + nbr * m.row_ns[t]) / 1e3;
-    const double c = std::max(mem_us, issue_us) + 3.0;  // + launch overhead
-    if (cost4_us) cost4_us[t] = c;
-    if ((allowed_mask >> t) & 1) {
-      if (c < best) {
-        best = c;
-        *T_out = T;
+  return bgb::host_guard([&]() -> int {
+    bgb::clear_error();
+    BGB_REQUIRE(op >= 0 && op <= 3 && nrows >= 0 && h_nblocks4 && T_out, BGB_ERR_ARG, "bad arguments");
+    if (op == 2) allowed_mask &= 0xC;  // the MFMA path takes T = 16, 32
+    BGB_REQUIRE(allowed_mask & 0xF, BGB_ERR_ARG, "no tile dim allowed");
+    const Model &m = kModels[op];
+    double best = 1e300;
+    for (int t = 0; t < 4; ++t) {
+      const int T = kTs[t];
+      const double nbr = (double)((nrows + T - 1) / T);
+      const double mem_us = op_bytes(op, T, nrows, h_nblocks4[t]) / (m.bw_gbs * 1e3);
+      const bool lds = op == 0 && T <= 16 && lds_x_bytes(T, nrows) <= kLdsForX;
+      const double tile_ns = lds ? kTileNsLds[t] : m.tile_ns[t];
+      const double issue_us = ((double)h_nblocks4[t] * tile_ns + nbr * m.row_ns[t]) / 1e3;
+      const double c = std::max(mem_us, issue_us) + 3.0;  // + launch overhead
+      if (cost4_us) cost4_us[t] = c;
+      if ((allowed_mask >> t) & 1) {
+        if (c < best) {
+          best = c;
+          *T_out = T;
+        }
       }
     }
-  }
-  return BGB_OK;
+    return BGB_OK;
+  });
 }
 
 }  // extern "C"

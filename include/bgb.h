@@ -112,11 +112,20 @@ int bgb_b2sr_transpose(int T, int nblockrows, int nblockcols, int nblocks, const
  * rowptr[nblockrows] (passed so that no device read is needed).             *
  * accumulate = 1 reproduces the reference's in-place accumulation           *
  * (|= for bits, += for counts and sums); 0 overwrites C.                    *
- * Outputs are 4-byte aligned; bit outputs (W_T[NBR]) are rounded up to a   *
- * whole number of dwords: pieces of rows that cross the kernels' tile      *
- * chunks are OR-ed / added atomically into them by a second, small pass.   *
+ * Bit outputs are exactly W_T[NBR] (the reference's allocation,            *
+ * test_bin_bin_bin.cu:329): no byte past C[NBR-1] is read or written.      *
+ * Outputs are 4-byte aligned (count / FP32 outputs 16-byte, T=64 bit       *
+ * outputs and masks 8-byte); hipMalloc gives 256.                          *
+ * x: a packed word (or T floats) for every block column the matrix         *
+ * references -- ceil(NBR/T)*T words for a square matrix, as the reference  *
+ * allocates them; a wide matrix (more block columns than block rows, e.g.  *
+ * the transpose of a tall one) needs one per block column.                 *
  * Calls on one stream are ordered; calls that overlap on different streams *
- * use separate per-stream carry workspaces.                                *
+ * use separate per-stream carry workspaces. These entry points keep a      *
+ * per-stream workspace and per-matrix facts (chunk partition, longest      *
+ * block row, largest block column -- the latter two measured on the first  *
+ * call for a matrix, which synchronises the stream), so they return        *
+ * BGB_ERR_ARG while their stream is being captured into a graph.          *
  * ------------------------------------------------------------------------ */
 
 /* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
@@ -138,8 +147,7 @@ int bgb_get_x_staging(int T);
  * fix-up). 0 = automatic: taken when no block row holds more than 32 tiles
  * (regular matrices such as 2-D grids); the figure is measured on the first
  * call for a (rowptr, nblockrows, nblocks) -- one reduction and a stream
- * synchronisation -- and kept; while the stream is being captured into a
- * graph it is not measured and the chunked kernels run. 1 = never, 2 =
+ * synchronisation -- and kept. 1 = never, 2 =
  * always (exact for any matrix, slow for long rows). Process-wide tuning
  * knob (no reference counterpart; the reference's kernels are one warp per
  * 1-8 block rows, bsrbmv.cu:298-303). Default 0. */
@@ -263,6 +271,16 @@ int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_
                       int nblockrows, int nblocks, const void *d_At, const int *d_rowptrT,
                       const int *d_colindT, int n, const int *h_sources, int nsrc, int *d_levels,
                       int use_graph, int *h_nlevels, bgb_stream_t stream);
+
+/* Measurement aid (no reference counterpart): with on = 1, calls of
+ * bgb_bfs_multi / bgb_bfs_multi_dir with use_graph = 0 run one level at a
+ * time between two HIP events (and read the level flags before each level,
+ * which synchronises the stream per level). Process-wide; default 0. */
+int bgb_set_bfs_level_timing(int on);
+/* The levels of the last timed call on this host thread: h_ms[i] = GPU time
+ * of level i in milliseconds, h_pulled[i] = 1 for a pull pass over the tiles,
+ * 0 for a push; *h_n = number of levels (at most cap are written). */
+int bgb_bfs_level_times(int cap, float *h_ms, int *h_pulled, int *h_n);
 
 /* One source: one masked bin_bin_bin BMV per level (frontier & ~visited),
  * the per-source loop the batched BFS is compared with. Synchronises once

@@ -35,10 +35,11 @@
  *                          and the number of those tiles whose row word is non-zero
  *   orc_estimate_tiles     estimatevec (utility.cu:955-975): per CSR row, the
  *                          number of runs of equal colind/T, summed
- *   orc_bfs, orc_pagerank_f64, orc_csr_spmm_f64
+ *   orc_bfs, orc_bfs_multi, orc_pagerank_f64, orc_csr_spmm_f64
  *                          build-only rows (multi-source BFS, PageRank, K-RHS)
- *                          with no reference counterpart: plain queue BFS, FP64
- *                          power iteration and FP64 CSR SpMM.
+ *                          with no reference counterpart: plain queue BFS (one
+ *                          per source, sources in parallel), FP64 power
+ *                          iteration and FP64 CSR SpMM.
  * How each is pinned (golden fixtures, dense numpy, scipy, the reference's own
  * readMtx.hpp built into oracle/_ref) is stated in the tests that use it.
  */
@@ -331,6 +332,19 @@ int orc_bfs(int n, const int *rowptr, const int *colind, int src, int *level) {
   }
   free(q);
   return depth + 1;
+}
+
+/* orc_bfs from each of nsrc sources, sources in parallel (OpenMP):
+ * level[s*n + v] = hop distance of v from srcs[s]; returns the largest
+ * number of levels over the sources. */
+int orc_bfs_multi(int n, const int *rowptr, const int *colind, int nsrc, const int *srcs, int *level) {
+  int most = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : most)
+  for (int s = 0; s < nsrc; ++s) {
+    const int nl = orc_bfs(n, rowptr, colind, srcs[s], level + (long long)s * n);
+    if (nl > most) most = nl;
+  }
+  return most;
 }
 
 /* pr_{t+1}[v] = (1-d)/n + d * (sum_{u in N(v)} pr_t[u]/deg(u) + dangling_t/n),

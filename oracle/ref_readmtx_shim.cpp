@@ -8,17 +8,27 @@
 
 #include "readMtx.hpp"
 
+// removeSelfloop's compaction (readMtx.hpp:202-204) reads the column vector
+// up to two slots past its end. Reserve those slots and fill the whole
+// capacity with -1 (an erased entry) beforehand, so what the reads see does
+// not depend on what the heap held there.
+template <class V>
+static void reserve_filled(std::vector<V> &v, size_t cap, V fill) {
+  const size_t n = v.size();
+  v.reserve(cap);
+  v.resize(cap, fill);
+  v.resize(n);  // capacity and the filled slots stay
+}
+
 extern "C" int ref_normalize(int nrows, int ncols, long long nedges, const int *src,
                              const int *dst, int undirected, int *rowptr, int *colind,
                              long long *nnz) {
   if (nedges <= 0) return -1;  // removeSelfloop reads element 0 unconditionally
   std::vector<int> r(src, src + nedges), c(dst, dst + nedges);
   std::vector<float> v(nedges, 1.0f);
-  // removeSelfloop's compaction may read one element past the end
-  // (readMtx.hpp:202-204); reserve so that slot is allocated memory.
-  r.reserve(2 * nedges + 1);
-  c.reserve(2 * nedges + 1);
-  v.reserve(2 * nedges + 1);
+  reserve_filled(r, 2 * nedges + 4, -1);
+  reserve_filled(c, 2 * nedges + 4, -1);
+  reserve_filled(v, 2 * nedges + 4, 1.0f);
   int nvals = (int)nedges;
   removeSelfloop<float>(&r, &c, &v, &nvals, undirected != 0);
   customSort<float>(&r, &c, &v);
@@ -36,9 +46,9 @@ extern "C" int ref_read_mtx(const char *path, int directed, int *nrows, int *nco
   std::vector<int> r, c;
   std::vector<float> v;
   int M = 0, N = 0, nv = 0;
-  r.reserve(2 * cap + 1);
-  c.reserve(2 * cap + 1);
-  v.reserve(2 * cap + 1);
+  reserve_filled(r, 2 * cap + 4, -1);
+  reserve_filled(c, 2 * cap + 4, -1);
+  reserve_filled(v, 2 * cap + 4, 1.0f);
   readMtx<float>(path, &r, &c, &v, &M, &N, &nv, directed, false, nullptr);
   if ((long long)r.size() > cap) return -1;
   std::vector<float> vals(r.size() > 0 ? r.size() : 1);

@@ -37,12 +37,49 @@ def to_np(t, T=None):
     return a
 
 
-def check_all_ops(orc, csr, T, rng, dev, label, xdensity=0.3):
-    """Packs csr on the GPU and checks every operator against the oracle."""
+GUARD_BYTES = 64
+GUARD_VALUE = 0xA5
+
+
+def dirty(n, dtype, dev):
+    """An output buffer of n elements with every byte 0xFF (int32 -1, float NaN):
+    an overwrite (accumulate=0) must replace every element the op defines."""
+    import torch
+    t = torch.empty(max(n, 1), dtype=dtype, device=dev)
+    t.view(torch.uint8).fill_(0xFF)
+    return t[:n]
+
+
+def guarded_bits(A, dev):
+    """(raw bytes, C): C is exactly NBR words of 0xFF bytes, the reference's
+    allocation (test_bin_bin_bin.cu:329), followed in the same allocation by
+    GUARD_BYTES bytes of GUARD_VALUE that no BMV may change."""
     import torch
     from bgb import b2sr as B
-    log(f"[case] {label} T={T} n={csr.nrows} nnz={csr.nnz}")
+    nb = A.nblockrows * B.lib().bgb_word_bytes(A.T)
+    raw = torch.full((nb + GUARD_BYTES,), GUARD_VALUE, dtype=torch.uint8, device=dev)
+    raw[:nb] = 0xFF
+    return raw, raw[:nb].view(B.WORD_TORCH[A.T])
+
+
+def check_guard(raw, A):
+    from bgb import b2sr as B
+    nb = A.nblockrows * B.lib().bgb_word_bytes(A.T)
+    tail = raw[nb:].cpu().numpy()
+    assert (tail == GUARD_VALUE).all(), "bytes past the bit output changed"
+
+
+def check_all_ops(orc, csr, T, rng, dev, label, xdensity=0.3):
+    """Packs csr on the GPU and checks every operator against the oracle.
+    Outputs start dirty (0xFF bytes) and are overwritten (accumulate=0); bit
+    outputs are exactly NBR words with a guard region behind them. x covers
+    every column (a wide matrix reads more block columns than it has block
+    rows)."""
+    import torch
+    from bgb import b2sr as B
+    log(f"[case] {label} T={T} n={csr.nrows}x{csr.ncols} nnz={csr.nnz}")
     n = csr.nrows
+    nx = max(csr.nrows, csr.ncols)
     A = B.B2SR.from_csr(csr, T, dev)
     brp, bci, val = orc.b2sr(T, n, csr.ncols, csr.rowptr, csr.colind)
     g_rp, g_ci, g_val = A.to_host()
@@ -50,35 +87,45 @@ def check_all_ops(orc, csr, T, rng, dev, label, xdensity=0.3):
     np.testing.assert_array_equal(g_ci, bci)
     np.testing.assert_array_equal(g_val, val)
     nbr = A.nblockrows
-    x01 = (rng.random(n) < xdensity).astype(np.float32)
+    x01 = (rng.random(nx) < xdensity).astype(np.float32)
     xpad = B.padded_vector(x01, T, dev)
     xb = B.pack_vec(xpad, T)
-    want_xb = orc.pack_vec(T, x01, nbr)
+    want_xb = orc.pack_vec(T, x01, -(-nx // T))
     np.testing.assert_array_equal(to_np(xb, T)[: want_xb.size], want_xb)
 
     counts = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
-    got = to_np(B.bmv_bin_bin_full(A, xb))
+    got = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(nbr * T, torch.int32, dev)))
     np.testing.assert_array_equal(got[:n], counts)
     assert not got[n:].any()
-    gotf = to_np(B.bmv_bin_bin_full(A, xb, dtype=torch.float32))
+    gotf = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(nbr * T, torch.float32, dev), dtype=torch.float32))
     np.testing.assert_array_equal(gotf[:n], counts.astype(np.float32))
+    assert not gotf[n:].any()
 
     want_bits = orc.pack_u8(T, (counts > 0).astype(np.uint8), nbr)
-    bits = to_np(B.bmv_bin_bin_bin(A, xb), T)
+    raw, C = guarded_bits(A, dev)
+    bits = to_np(B.bmv_bin_bin_bin(A, xb, out=C), T)
     np.testing.assert_array_equal(bits, want_bits)
+    check_guard(raw, A)
 
     mbits = (rng.random(n) < 0.5).astype(np.uint8)
     mask = torch.from_numpy(orc.pack_u8(T, mbits, nbr).view(to_np(xb).dtype)).to(dev)
-    got_m = to_np(B.bmv_bin_bin_bin_masked(A, xb, mask), T)
+    raw, C = guarded_bits(A, dev)
+    got_m = to_np(B.bmv_bin_bin_bin_masked(A, xb, mask, out=C), T)
     want_m = orc.pack_u8(T, ((counts > 0) & (mbits == 0)).astype(np.uint8), nbr)
     np.testing.assert_array_equal(got_m, want_m)
+    check_guard(raw, A)
 
-    xf = rng.random(n).astype(np.float32)
+    xf = rng.random(nx).astype(np.float32)
     xfp = B.padded_vector(xf, T, dev)
-    gff = to_np(B.bmv_bin_full_full(A, xfp))
+    gff = to_np(B.bmv_bin_full_full(A, xfp, out=dirty(nbr * T, torch.float32, dev)))
     ref64 = orc.csr_f64(csr.rowptr, csr.colind, xf.astype(np.float64))
     np.testing.assert_allclose(gff[:n], ref64, rtol=FP_RTOL, atol=FP_ATOL)
     assert not gff[n:].any()
+    xh = xfp.to(torch.bfloat16)
+    want_h = orc.csr_f64(csr.rowptr, csr.colind, xh.float().cpu().numpy()[:nx].astype(np.float64))
+    gh = to_np(B.bmv_bin_full_full_bf16(A, xh, out=dirty(nbr * T, torch.float32, dev)))
+    np.testing.assert_allclose(gh[:n], want_h, rtol=FP_RTOL, atol=FP_ATOL)
+    assert not gh[n:].any()
     return A, xb, xfp
 
 

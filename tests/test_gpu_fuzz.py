@@ -12,7 +12,7 @@ accumulate=1 adds onto the output. Bit and integer results must be exact.
 import numpy as np
 import pytest
 
-from test_gpu_bmv import check_all_ops, to_np
+from test_gpu_bmv import check_all_ops, dirty, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -34,8 +34,8 @@ def random_graph(rng):
         src = np.concatenate([rng.integers(0, max(a, 1), m), rng.integers(b, n, m)])
         dst = np.concatenate([rng.integers(0, max(a, 1), m), rng.integers(b, n, m)])
         return "holes", graph.from_coo(n, n, src, dst, True, True)
-    if kind == 2:  # non-square (m <= n: x is drawn per row and covers every column), directed
-        m = int(rng.integers(min(20, n), n + 1))
+    if kind == 2:  # non-square, tall or wide (x covers every column), directed
+        m = int(rng.integers(min(20, n), 2 * n + 1))
         e = int(rng.integers(1, 12)) * max(n, m)
         return "rect", graph.from_coo(n, m, rng.integers(0, n, e), rng.integers(0, m, e), False, False)
     if kind == 3:  # near-regular short rows (a grid)
@@ -123,12 +123,14 @@ def test_random_k16(orc, cuda):
         X = torch.from_numpy(rng.uniform(-1, 1, (csr.nrows, 16)).astype(np.float16)).to(cuda)
         X64 = X.cpu().numpy().astype(np.float64)
         Xb = B.k16_pack_rhs(X, A.nblockrows * T)
-        check(csr, B.bmv_k16(A, Xb), X64, f"seed {seed} {label} T={T}")
+        Y = B.bmv_k16(A, Xb, out=dirty(A.nblockrows * T * 16, torch.float32, cuda))
+        check(csr, Y, X64, f"seed {seed} {label} T={T}")
         mats.append(A)
         xbs.append(Xb)
         refs.append((csr, X64))
     assert len(refs) >= 20
     batch = B.K16Batch(mats, xbs, nworkgroups=96)
+    batch.y.view(torch.uint8).fill_(0xFF)
     batch.run()
     for i, (csr, X64) in enumerate(refs):
         check(csr, batch.result(i), X64, f"batch graph {i}")

@@ -10,6 +10,8 @@ import sys
 import numpy as np
 import pytest
 
+from test_gpu_bmv import dirty
+
 pytestmark = pytest.mark.gpu
 
 
@@ -45,6 +47,7 @@ def check(orc, csr, Y, X64, label):
 
 @pytest.mark.parametrize("T", (16, 32))
 def test_k16_single(orc, dev, T):
+    import torch
     from bgb import b2sr as B
     from bgb import graph
     for label, csr in (("rmat13", graph.rmat(13, 8, seed=3)), ("er3000", graph.erdos_renyi(3000, 20, 5)),
@@ -52,7 +55,7 @@ def test_k16_single(orc, dev, T):
         A = B.B2SR.from_csr(csr, T, dev)
         X, X64 = rhs(csr.nrows, T, dev)
         Xb = B.k16_pack_rhs(X, A.nblockrows * T)
-        Y = B.bmv_k16(A, Xb)
+        Y = B.bmv_k16(A, Xb, out=dirty(A.nblockrows * T * 16, torch.float32, dev))  # overwritten
         check(orc, csr, Y, X64, f"{label} T={T}")
         Y2 = B.bmv_k16(A, Xb, out=Y.clone(), accumulate=True)
         np.testing.assert_allclose(Y2.cpu().numpy(), 2 * Y.cpu().numpy(), rtol=1e-6, atol=1e-6)
@@ -60,7 +63,9 @@ def test_k16_single(orc, dev, T):
 
 @pytest.mark.timeout(120)
 def test_k16_persistent_batch(orc, dev):
-    """Graphs of uneven sizes (log-uniform n), half ER, half R-MAT, mixed T, one launch."""
+    """Graphs of uneven sizes (log-uniform n), half ER, half R-MAT, tile dim from
+    the autotuner, one launch."""
+    import torch
     from bgb import b2sr as B
     from bgb import graph
     rng = np.random.default_rng(11)
@@ -71,7 +76,7 @@ def test_k16_persistent_batch(orc, dev):
             csr = graph.erdos_renyi(max(n, 64), int(rng.integers(8, 33)), seed=i)
         else:
             csr = graph.rmat(max(7, int(np.log2(max(n, 128)))), 8, seed=i)
-        T = 16 if (csr.nnz / max(csr.nrows, 1)) < 12 else 32
+        T = graph.autotune_tiledim(csr, "k16", allowed=(16, 32))[0]
         A = B.B2SR.from_csr(csr, T, dev)
         X, X64 = rhs(csr.nrows, 100 + i, dev)
         mats.append(A)
@@ -79,6 +84,7 @@ def test_k16_persistent_batch(orc, dev):
         refs.append((csr, X64, T))
     batch = B.K16Batch(mats, xbs, nworkgroups=64)
     log(f"[case] k16 persistent batch: {len(mats)} graphs, 64 workgroups")
+    batch.y.view(torch.uint8).fill_(0xFF)  # the launch overwrites every output row
     batch.run()
     for i, (csr, X64, T) in enumerate(refs):
         check(orc, csr, batch.result(i), X64, f"batch graph {i} n={csr.nrows} T={T}")
@@ -89,6 +95,7 @@ def test_k16_batch_rmat22(orc, dev):
     """Three R-MAT scale-22 graphs at T=16 in one persistent launch (about a
     million 256-tile chunks; hub rows carry across many chunks and across the
     fix-up's 64-chunk waves): every graph within the FP32 bound."""
+    import torch
     from bgb import b2sr as B
     from bgb import graph
     mats, xbs, refs = [], [], []
@@ -103,6 +110,7 @@ def test_k16_batch_rmat22(orc, dev):
         chunks += -(-A.nblocks // 256)
     log(f"[case] k16 batch of 3 R-MAT 22 graphs: {chunks} chunks")
     batch = B.K16Batch(mats, xbs)
+    batch.y.view(torch.uint8).fill_(0xFF)
     batch.run()
     for i, (csr, X64) in enumerate(refs):
         check(orc, csr, batch.result(i), X64, f"rmat22 #{i}")
