@@ -51,6 +51,7 @@ SIGNATURES = {
     "bgb_get_short_row_path": (_c_int, []),
     "bgb_bmv_bin_bin_bin": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_full_i32": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
+    "bgb_bmv_bin_bin_full_i32_noalloc": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_full_f32": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_full_full": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_bin_masked": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp]),

@@ -150,9 +150,13 @@ def bmv_bin_bin_bin(A, xb, out=None, accumulate=False):
     return C
 
 
-def bmv_bin_bin_full(A, xb, out=None, accumulate=False, dtype=torch.int32):
+def bmv_bin_bin_full(A, xb, out=None, accumulate=False, dtype=torch.int32, noalloc=False):
+    """Neighbour counts; noalloc=True calls the allocation-free int32 entry point."""
     C = _out(out, A.nblockrows * A.T, dtype, A.device, accumulate)
     fn = "bgb_bmv_bin_bin_full_i32" if dtype == torch.int32 else "bgb_bmv_bin_bin_full_f32"
+    if noalloc:
+        assert dtype == torch.int32
+        fn = "bgb_bmv_bin_bin_full_i32_noalloc"
     call(fn, A.T, _ptr(A.val), _ptr(xb), _ptr(C), _ptr(A.rowptr), _ptr(A.colind), A.nblockrows,
          A.nblocks, int(accumulate), _stream())
     return C

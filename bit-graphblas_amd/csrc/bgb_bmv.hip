@@ -838,8 +838,47 @@ __global__ void fill_kernel(V *p, long long n, V v) {
     p[i] = v;
 }
 
+// Scratch of the allocation-free entry point (bgb_bmv_bin_bin_full_i32_noalloc):
+// carry keys, chunk partition and carry records of one call, 64 MiB per
+// device, held from the library's load until the process ends. Every call on
+// a device shares that device's copy, so such calls must not overlap on one
+// device (they would race on it); nothing in it needs clearing between calls
+// (every chunk writes its key and partition entry, records are read only
+// under a key that says they were written). Host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device: the name
+// itself is a host-side stand-in, not a device address.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices];  // per device, set on its first call
+}  // namespace
+
+// Device address of the current device's copy of `scratch`, looked up once
+// per device (with that device current) and kept.
+int static_scratch(uint8_t **out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  BGB_REQUIRE(dev >= 0 && dev < kMaxDevices, BGB_ERR_ARG, "device %d beyond the %d the scratch table holds", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    BGB_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BGB_REQUIRE(p, BGB_ERR_HIP, "hipGetSymbolAddress returned no address for the static scratch");
+    g_scratch_ptr[dev] = static_cast<uint8_t *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BGB_OK;
+}
+
+// scr: the static scratch of the allocation-free entry point (carry records
+// and a partition made on every call, x gathered from memory), or null for
+// the per-stream workspace, the cached partition and the automatic paths.
 template <int T, int OP, int S>
-int launch_bmv_s(BmvArgs a, hipStream_t s) {
+int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   constexpr int CH = (64 / Tile<T>::L) * S;
   using Acc = typename AccOf<OP>::type;
   const long long nchunks = ceil_div(a.nblocks, CH);
@@ -848,11 +887,23 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
   void *ws = nullptr;
   const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
   const size_t val_bytes = (sizeof(Acc) * (size_t)nchunks * T + 255) & ~(size_t)255;
-  int st = stream_workspace(s, key_bytes + val_bytes, &ws);
-  if (st) return st;
-  a.carry_key = static_cast<int *>(ws);
-  a.carry_val = static_cast<char *>(ws) + key_bytes;
-  if ((st = chunk_partition(s, a.rowptr, a.nbr, a.nblocks, CH, &a.part))) return st;
+  int st = 0;
+  if (scr) {
+    const size_t need = 2 * key_bytes + val_bytes;  // keys, partition, records
+    BGB_REQUIRE(need <= kScratchBytes, BGB_ERR_SIZE,
+                "%lld tiles at T=%d need %zu bytes of scratch; the static scratch holds %zu", (long long)a.nblocks,
+                T, need, kScratchBytes);
+    a.carry_key = reinterpret_cast<int *>(scr);
+    a.part = reinterpret_cast<int *>(scr + key_bytes);
+    a.carry_val = scr + 2 * key_bytes;
+    chunk_rows_kernel<<<(unsigned)ceil_div(a.nbr, 256), 256, 0, s>>>(a.rowptr, a.nbr, CH, (int)nchunks, a.part);
+    BGB_HIP_TRY(hipGetLastError());
+  } else {
+    if ((st = stream_workspace(s, key_bytes + val_bytes, &ws))) return st;
+    a.carry_key = static_cast<int *>(ws);
+    a.carry_val = static_cast<char *>(ws) + key_bytes;
+    if ((st = chunk_partition(s, a.rowptr, a.nbr, a.nblocks, CH, &a.part))) return st;
+  }
   bool lds_path = false;
   if constexpr (!is_full(OP)) {
     // Whole packed x in LDS when it fits and the tile stream is large next to
@@ -862,7 +913,7 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
     if ((st = device_facts(&f))) return st;
     int maxcol = -1;
     const int mode = g_x_staging[ilog2_tiledim(T) - 2];
-    if (mode != 1 && (mode == 2 || T <= 16))
+    if (!scr && mode != 1 && (mode == 2 || T <= 16))
       if ((st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
     const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
     const long long x_bytes = words * (long long)sizeof(typename Tile<T>::W);
@@ -905,10 +956,11 @@ int launch_bmv_s(BmvArgs a, hipStream_t s) {
 }
 
 template <int T, int OP>
-int launch_bmv(BmvArgs a, hipStream_t s) {
+int launch_bmv(BmvArgs a, hipStream_t s, uint8_t *scr) {
   if constexpr (Tile<T>::L == 1) {
-    int mr = -1;  // most tiles in a block row; -1: unknown
-    if (g_short_rows == 2) {
+    int mr = -1;  // most tiles in a block row; -1: unknown (always with scr)
+    if (scr) {
+    } else if (g_short_rows == 2) {
       mr = 0;
     } else if (g_short_rows == 0) {
       const int st = max_row_tiles(s, a.rowptr, a.nbr, a.nblocks, &mr);
@@ -926,12 +978,15 @@ int launch_bmv(BmvArgs a, hipStream_t s) {
       return BGB_OK;
     }
   }
-  return g_run_length[ilog2_tiledim(T) - 2] == 16 ? launch_bmv_s<T, OP, 16>(a, s) : launch_bmv_s<T, OP, 8>(a, s);
+  return g_run_length[ilog2_tiledim(T) - 2] == 16 ? launch_bmv_s<T, OP, 16>(a, s, scr)
+                                                  : launch_bmv_s<T, OP, 8>(a, s, scr);
 }
 
+// noalloc: the allocation-free variant (static scratch, no caches, nothing
+// allocated or synchronised).
 template <int OP>
 int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, const int *colind,
-              int nbr, int nblocks, int accumulate, const void *mask, bgb_stream_t stream) {
+              int nbr, int nblocks, int accumulate, const void *mask, bgb_stream_t stream, bool noalloc = false) {
   return host_guard([&]() -> int {
     clear_error();
     BGB_REQUIRE(valid_tiledim(T), BGB_ERR_TILEDIM, "tile dim %d not in {4,8,16,32,64}", T);
@@ -952,13 +1007,20 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
                 BGB_ERR_ARG, "T=64 bit outputs and masks must be 8-byte aligned");
     if (nbr == 0) return BGB_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    // a captured call would keep pointers to the per-stream workspace and
-    // the cached partition, which later calls may free or reuse
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    BGB_HIP_TRY(hipStreamIsCapturing(s, &cap));
-    BGB_REQUIRE(cap == hipStreamCaptureStatusNone, BGB_ERR_ARG,
-                "BMV entry points cannot run while their stream is being captured (their carry workspace and "
-                "chunk partition are library-cached and may be freed or reused by later calls)");
+    uint8_t *scr = nullptr;
+    if (noalloc) {
+      int st = static_scratch(&scr);
+      if (st) return st;
+    } else {
+      // a captured call would keep pointers to the per-stream workspace and
+      // the cached partition, which later calls may free or reuse
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      BGB_HIP_TRY(hipStreamIsCapturing(s, &cap));
+      BGB_REQUIRE(cap == hipStreamCaptureStatusNone, BGB_ERR_ARG,
+                  "this BMV entry point cannot run while its stream is being captured (its carry workspace and "
+                  "chunk partition are library-cached and may be freed or reused by later calls); "
+                  "bgb_bmv_bin_bin_full_i32_noalloc can");
+    }
     if (nblocks == 0) {  // every block row is empty
       if (accumulate && OP != OP_MASKED) return BGB_OK;
       const long long n = (OP == OP_BITS || OP == OP_MASKED) ? (long long)nbr * word_bytes(T)
@@ -969,11 +1031,11 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
     }
     BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr};
     switch (T) {
-      case 4: return launch_bmv<4, OP>(a, s);
-      case 8: return launch_bmv<8, OP>(a, s);
-      case 16: return launch_bmv<16, OP>(a, s);
-      case 32: return launch_bmv<32, OP>(a, s);
-      default: return launch_bmv<64, OP>(a, s);
+      case 4: return launch_bmv<4, OP>(a, s, scr);
+      case 8: return launch_bmv<8, OP>(a, s, scr);
+      case 16: return launch_bmv<16, OP>(a, s, scr);
+      case 32: return launch_bmv<32, OP>(a, s, scr);
+      default: return launch_bmv<64, OP>(a, s, scr);
     }
   });
 }
@@ -1264,6 +1326,13 @@ int bgb_bmv_bin_bin_full_f32(int T, const void *d_A, const void *d_B, float *d_C
                              int accumulate, bgb_stream_t stream) {
   return bmv_entry<OP_COUNT_F32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks,
                                  accumulate, nullptr, stream);
+}
+
+int bgb_bmv_bin_bin_full_i32_noalloc(int T, const void *d_A, const void *d_B, int *d_C, const int *d_rowptr,
+                                     const int *d_colind, int nblockrows, int nblocks, int accumulate,
+                                     bgb_stream_t stream) {
+  return bmv_entry<OP_COUNT_I32>(T, d_A, d_B, d_C, d_rowptr, d_colind, nblockrows, nblocks, accumulate,
+                                 nullptr, stream, /*noalloc=*/true);
 }
 
 int bgb_bmv_bin_full_full(int T, const void *d_A, const float *d_B, float *d_C, const int *d_rowptr,

@@ -125,7 +125,8 @@ int bgb_b2sr_transpose(int T, int nblockrows, int nblockcols, int nblocks, const
  * per-stream workspace and per-matrix facts (chunk partition, longest      *
  * block row, largest block column -- the latter two measured on the first  *
  * call for a matrix, which synchronises the stream), so they return        *
- * BGB_ERR_ARG while their stream is being captured into a graph.          *
+ * BGB_ERR_ARG while their stream is being captured into a graph;           *
+ * bgb_bmv_bin_bin_full_i32_noalloc is the capturable form.                 *
  * ------------------------------------------------------------------------ */
 
 /* Tiles each lane group walks per chunk (8 or 16) for tile dim T; process-wide
@@ -166,6 +167,22 @@ int bgb_bmv_bin_bin_full_i32(int T, const void *d_A, const void *d_B, int *d_C,
                              const int *d_rowptr, const int *d_colind,
                              int nblockrows, int nblocks, int accumulate,
                              bgb_stream_t stream);
+
+/* bgb_bmv_bin_bin_full_i32 without any allocation (no reference counterpart;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results. Its scratch (carry records and the chunk partition,
+ * rebuilt on every call) is a static 64 MiB device array per GPU, held from
+ * the library's load until the process ends: BGB_ERR_SIZE when a matrix
+ * needs more (about 2.6 M tiles at T=32, 12 M at T=16, 90 M at T=4/8).
+ * It calls no hipMalloc / hipFree, never synchronises and caches nothing, so
+ * it always takes the chunked kernels with x gathered from memory. Every call
+ * on a GPU shares that GPU's scratch and it keeps the last call's contents:
+ * calls that overlap on one GPU (different streams, or graph replays in
+ * flight at once) race and must not be issued. */
+int bgb_bmv_bin_bin_full_i32_noalloc(int T, const void *d_A, const void *d_B, int *d_C,
+                                     const int *d_rowptr, const int *d_colind, int nblockrows,
+                                     int nblocks, int accumulate, bgb_stream_t stream);
 
 /* Same counts stored as float, the reference's output type
  * (test_bin_bin_full.cu:62,314,335-346). Exact below 2^24. */
