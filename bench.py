@@ -285,18 +285,18 @@ def bfs_section(torch, B, graph, csr, T, nsrc=64, reps=3, plain=True):
     pb = bfs_pull_bytes(T, A.nblockrows, A.nblocks)
     per_level = []
     for i, (ms, pulled) in enumerate(levels):
-        e = {"level": i, "ms": round(ms, 4), "pass": "pull" if pulled else "push"}
+        rec = {"level": i, "ms": round(ms, 4), "pass": "pull" if pulled else "push"}
         if pulled:
-            e.update({"alg_bytes": pb, "alg_gbs": round(pb / (ms / 1e3) / 1e9, 1),
-                      "hbm_frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
-        per_level.append(e)
+            rec.update({"alg_bytes": pb, "alg_gbs": round(pb / (ms / 1e3) / 1e9, 1),
+                        "hbm_frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+        per_level.append(rec)
     out["per_level"] = per_level
     out["per_level_identical"] = bool(torch.equal(lev_t, lev))
-    pulls = [e for e in per_level if e["pass"] == "pull"]
+    pulls = [r for r in per_level if r["pass"] == "pull"]
     if pulls:
         out["pull_roofline"] = {"bound": "hbm", "alg_bytes_per_level": pb,
-                                "best_frac": max(e["hbm_frac"] for e in pulls),
-                                "mean_ms": round(sum(e["ms"] for e in pulls) / len(pulls), 4)}
+                                "best_frac": max(r["hbm_frac"] for r in pulls),
+                                "mean_ms": round(sum(r["ms"] for r in pulls) / len(pulls), 4)}
     log(f"[bench] BFS per level: {per_level}")
     del lev_t
     if plain:
@@ -543,9 +543,10 @@ def main():
     ap.add_argument("--x-staging", type=int, default=-1,
                     help="packed-x staging of the bit BMVs for every tile dim: 0 auto, 1 never (gathered "
                          "from memory), 2 in LDS whenever it fits; -1 keeps the library default")
-    ap.add_argument("--profile", action="store_true",
-                    help="profiling workload only: a 256 MiB copy (counter calibration) then --steps BMVs "
-                         "at the first of --tiles")
+    ap.add_argument("--profile", nargs="?", const="bmv", default=None, choices=("bmv", "bfs", "k16"),
+                    help="profiling workload only: a 256 MiB copy (counter calibration), then --steps BMVs "
+                         "at the first of --tiles on --config (bmv), one eager 64-source BFS on R-MAT 22 "
+                         "(bfs), or --steps K=16 MFMA calls on R-MAT 20 (k16)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -583,6 +584,33 @@ def main():
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        if args.profile == "bfs":
+            csr22 = graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1)
+            A = B.B2SR.from_csr(csr22, T, dev)
+            rng = np.random.default_rng(64)
+            src = rng.choice(np.nonzero(csr22.degrees())[0], 64, replace=False).astype(np.int32)
+            lev, nl, levels = B.bfs_level_times(A, src, push=A)
+            pb = bfs_pull_bytes(T, A.nblockrows, A.nblocks)
+            log(f"[profile] bfs T={T} nblocks={A.nblocks} pull_alg_bytes={pb} levels={levels}")
+            print(json.dumps({"profile": "bfs", "tile_dim": T, "pull_alg_bytes": pb, "nblocks": A.nblocks,
+                              "levels": levels}), flush=True)
+            return
+        if args.profile == "k16":
+            T = 16 if T not in (16, 32) else T
+            g = torch.Generator(device=dev)
+            g.manual_seed(16)
+            X = (torch.rand((csr.nrows, 16), generator=g, device=dev) * 2 - 1).to(torch.float16)
+            A = B.B2SR.from_csr(csr, T, dev)
+            Xb = B.k16_pack_rhs(X, A.nblockrows * T)
+            Y = torch.empty((A.nblockrows * T, 16), dtype=torch.float32, device=dev)
+            t, _ = time_bmv(torch, lambda: B.bmv_k16(A, Xb, out=Y), args.steps, 1)
+            wb = {16: 2, 32: 4}[T]
+            nbytes = (4 * (A.nblockrows + 1) + A.nblocks * (4 + T * wb) + Xb.numel() * 2
+                      + 4 * 16 * A.nblockrows * T)
+            log(f"[profile] k16 T={T} nblocks={A.nblocks} alg_bytes={nbytes} event_ms={t * 1e3:.4f}")
+            print(json.dumps({"profile": "k16", "tile_dim": T, "alg_bytes": nbytes, "event_ms": t * 1e3,
+                              "nblocks": A.nblocks, "nnz": csr.nnz}), flush=True)
+            return
         A = B.B2SR.from_csr(csr, T, dev)
         xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
         C = torch.empty(A.nblockrows * T, dtype=torch.int32, device=dev)
