@@ -78,7 +78,20 @@ struct BfsState {
   const int *colindT;
   int *big_n;            // push: entries in big_list
   int2 *big_list;        // push: frontier rows (I, r) of block rows longer than kPushSeg tiles
+  uint64_t *live;        // sources whose frontier F is non-empty (this level's input)
+  unsigned long long *live_acc;  // kCounters words: OR of this level's new bits, spread
+  uint8_t *sat;          // pull levels: block row I saturated (every vertex reached by every live source)
+  uint8_t *skip;         // pull levels: every block row of chunk c saturated
 };
+
+// ORs a wave's new bits into one of the kCounters spread words (no returned
+// value, so the wave does not wait for it); the end-of-level kernel folds
+// them into the next level's live mask.
+__device__ __forceinline__ void publish_live(const BfsState &b, uint64_t ln, int lane, long long slot) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) ln |= __shfl_xor(ln, m);
+  if (lane == 0 && ln) atomicOr(b.live_acc + (slot & (kCounters - 1)), (unsigned long long)ln);
+}
 
 // Records the vertices first reached at this level (new bits) for one row.
 __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint64_t vold, uint64_t nb,
@@ -97,7 +110,7 @@ __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint6
 template <int T>
 __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const uint64_t (&acc)[Tile<T>::R],
                                           const uint64_t (&vr)[Tile<T>::R], int depth, bool &any,
-                                          unsigned long long &cost) {
+                                          unsigned long long &cost, uint64_t &lnew) {
   constexpr int R = Tile<T>::R, L = Tile<T>::L;
   using W = typename Tile<T>::W;
   uint32_t bits = 0;
@@ -105,6 +118,7 @@ __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const
   for (int i = 0; i < R; ++i) {
     const long long v = (long long)I * T + q * R + i;
     const uint64_t nb = acc[i] & ~vr[i];
+    lnew |= nb;
     if (record_new(b, v, vr[i], nb, depth)) {
       any = true;
       bits |= 1u << (T - 1 - (q * R + i));
@@ -135,8 +149,6 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves +
                                                    (threadIdx.x >> 6));
   if (chunk >= b.nchunks) return;
-  const int Ia = b.part[chunk];
-  const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
@@ -147,15 +159,29 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   int J[S];
   const int n = (int)(pb - pa);
   if (active) load_run<T, S>(b.A, b.colind, pa, n, q, F, J);
+  // Every block row of the chunk saturated (bfs_chunk_skip_kernel): nothing
+  // can be new here and the rows' next-frontier words are already cleared.
+  // The flag is a scalar load: the test waits for it, not for the tiles
+  // already in flight (dropped when the chunk stops here).
+  if (b.skip[chunk]) {
+    if (lane == 0) b.carry_key[chunk] = -1;
+    return;
+  }
+  const int Ia = b.part[chunk];
+  const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
+  const uint64_t dead = ~*b.live;  // sources whose BFS has no frontier left
+  RowWindow rw;
+  int wv[kWin / 64];
   // frontier-non-empty words of the run's tiles, gathered unconditionally
   // (block column 0 for missing tiles, then zeroed) so the loads issue together
   uint32_t X[S];
+  auto gather_x = [&]() {
 #pragma unroll
-  for (int s = 0; s < S; ++s) X[s] = (uint32_t) static_cast<const W *>(b.fnz)[(active && s < n) ? J[s] : 0];
+    for (int s = 0; s < S; ++s) X[s] = (uint32_t) static_cast<const W *>(b.fnz)[(active && s < n) ? J[s] : 0];
 #pragma unroll
-  for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : 0u;
-  RowWindow rw;
-  int wv[kWin / 64];
+    for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : 0u;
+  };
+  gather_x();
   RowWindow::load(rowptr, Ia, Ib, lane, wv);
   rw.put(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane, wv);
   // the walk, compiled for row ends in the LDS window and in memory
@@ -169,14 +195,10 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 
   bool any = false;
   unsigned long long nnew = 0;  // push cost of the rows this wave adds
-  uint64_t acc[R], zero[R], vr[R], hv[R], hvr[R];
+  uint64_t lnew = 0;            // sources with a new bit in this wave's rows
+  uint64_t acc[R], vr[R], hv[R], hvr[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) acc[i] = zero[i] = hv[i] = hvr[i] = 0;
-  if (active && pa == 0)
-    for (int r = 0; r < I0; ++r) {
-      load_visited<T>(b, r, q, vr);
-      bfs_flush<T>(b, r, q, zero, vr, depth, any, nnew);
-    }
+  for (int i = 0; i < R; ++i) acc[i] = hv[i] = hvr[i] = 0;
   if (active) load_visited<T>(b, I, q, vr);
   int held = -1;
   // end of row I and, read one row ahead, of row I+1 (indices clamped into the
@@ -192,10 +214,11 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         uint32_t w = row_word<T>(F[s], i) & X[s];
-        // Stop once the partial covers every source that has not reached the
-        // vertex yet: more ORs cannot change acc & ~visited (this includes
-        // vertices every source has reached).
-        while (w && (acc[i] | vr[i]) != b.full) {
+        // Stop once the partial covers every live source that has not
+        // reached the vertex yet: frontier words only hold live sources, so
+        // more ORs cannot change acc & ~visited (this includes vertices every
+        // live source has reached).
+        while (w && (acc[i] | vr[i] | dead) != ~0ull) {
           const int h = 31 - __clz(w);
           acc[i] |= fr[T - 1 - h];
           w &= ~(1u << h);
@@ -211,18 +234,30 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
             hvr[i] = vr[i];
           }
         } else {
-          bfs_flush<T>(b, I, q, acc, vr, depth, any, nnew);
+          bfs_flush<T>(b, I, q, acc, vr, depth, any, nnew, lnew);
         }
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
         ++I;
         e = e2;
         e2 = row_end(I + 1);
-        while (e == pos) {
-          load_visited<T>(b, I, q, vr);
-          bfs_flush<T>(b, I, q, zero, vr, depth, any, nnew);
-          ++I;
-          e = e2;
+        if (e == pos) {
+          // block rows without tiles follow (their words were cleared by the
+          // level's first pass): step to the first row ending after pos, by a
+          // binary search over the row ends when more than one is empty
+          if (e2 > pos) {
+            ++I;
+            e = e2;
+          } else {
+            int lo = I + 2, up = ST ? (rw.Ib + 1 < b.nbr ? rw.Ib + 1 : b.nbr) : b.nbr;
+            while (lo < up) {
+              const int mid = (lo + up) >> 1;
+              if (row_end(mid) > pos) up = mid;
+              else lo = mid + 1;
+            }
+            I = lo;
+            e = row_end(I);
+          }
           e2 = row_end(I + 1);
         }
         if (I < b.nbr) load_visited<T>(b, I, q, vr);
@@ -240,7 +275,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     const uint64_t up = group_up<L, uint64_t>(sc[i]);
     if (g > 0 && kin == I0) hv[i] |= up;
   }
-  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any, nnew);
+  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any, nnew, lnew);
   const int last = (int)lmin2(G - 1, (b.nblocks - 1 - k0) / S);
   if (g == last) {
     const bool open = key < b.nbr;
@@ -251,6 +286,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     }
   }
   if (__ballot(any) != 0 && lane == 0) b.ctl[1] = 1;
+  publish_live(b, lnew, lane, chunk);
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) nnew += __shfl_xor(nnew, m);
   if (lane == 0 && nnew) atomicAdd(b.fcnt + (chunk & (kCounters - 1)), nnew);
@@ -277,6 +313,7 @@ __global__ __launch_bounds__(256) void bfs_fixup_kernel(BfsState b) {
   const bool last_of_run = lane == 63 || next != key;
   const int depth = b.ctl[0];
   bool any = false;
+  uint64_t lnew = 0;
   for (int r = 0; r < T; ++r) {
     uint64_t sc[1] = {key != INT_MAX ? b.carry_val[c * T + r] : 0ull};
     group_scan<1, 1, uint64_t>(key, sc, [](uint64_t x, uint64_t y) { return x | y; });
@@ -284,6 +321,7 @@ __global__ __launch_bounds__(256) void bfs_fixup_kernel(BfsState b) {
     const long long v = (long long)key * T + r;
     const uint64_t nb = sc[0] & ~b.V[v];
     if (!nb) continue;
+    lnew |= nb;
     const unsigned long long fold = atomicOr(reinterpret_cast<unsigned long long *>(b.Fn + v), (unsigned long long)nb);
     if (!fold && b.rowptrT)
       atomicAdd(b.fcnt + (c & (kCounters - 1)), (unsigned long long)(b.rowptrT[key + 1] - b.rowptrT[key]));
@@ -295,37 +333,93 @@ __global__ __launch_bounds__(256) void bfs_fixup_kernel(BfsState b) {
     any = true;
   }
   if (any) b.ctl[1] = 1;
+  publish_live(b, lnew, lane, c >> 6);
 }
 
 // End of a level: stop when nothing new was found, else advance the depth and
 // pick the next level's direction from the push cost of the vertices this
 // level added (push below push_limit). One block of kCounters threads.
-__global__ void bfs_advance_kernel(int *ctl, unsigned long long *fcnt, unsigned long long push_limit) {
-  __shared__ unsigned long long part[kCounters];
+// The sources whose BFS found new vertices (the OR of the spread live words)
+// are the next level's live mask.
+__global__ void bfs_advance_kernel(int *ctl, unsigned long long *fcnt, unsigned long long push_limit,
+                                   unsigned long long *live_acc, uint64_t *live) {
+  __shared__ unsigned long long part[kCounters], lor[kCounters];
   const int t = threadIdx.x;
   part[t] = fcnt[t];
   fcnt[t] = 0;
+  lor[t] = live_acc[t];
+  live_acc[t] = 0;
   __syncthreads();
   for (int m = kCounters / 2; m > 0; m >>= 1) {
-    if (t < m) part[t] += part[t + m];
+    if (t < m) {
+      part[t] += part[t + m];
+      lor[t] |= lor[t + m];
+    }
     __syncthreads();
   }
   if (t != 0 || ctl[2]) return;
+  *live = lor[0];
   if (ctl[1] == 0) ctl[2] = 1;
   else ctl[0] += 1;
   ctl[1] = 0;
   ctl[3] = part[0] < push_limit ? 1 : 0;
 }
 
-// Push levels write the next frontier and its packed non-empty bits with
-// atomics: both start cleared (a pull level overwrites every entry instead).
+// Start of a level. Push levels write the next
+// frontier and its packed non-empty bits with atomics: both start cleared.
+// A pull level overwrites every entry of the non-empty rows it walks; here it
+// clears the next-frontier words of the block rows without tiles and of the
+// saturated ones -- every vertex already reached by every live source, so
+// nothing can be new there -- and marks the latter, so the level kernel may
+// skip chunks made of such rows.
+template <int T>
 __global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
-  if (b.ctl[2] || !b.ctl[3]) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *b.big_n = 0;
+  using W = typename Tile<T>::W;
+  if (b.ctl[2]) return;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < b.npad; i += stride) b.Fn[i] = 0;
-  uint8_t *z = static_cast<uint8_t *>(b.fnzn);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < zbytes; i += stride) z[i] = 0;
+  if (b.ctl[3]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.big_n = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < b.npad; i += stride) b.Fn[i] = 0;
+    uint8_t *z = static_cast<uint8_t *>(b.fnzn);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < zbytes; i += stride) z[i] = 0;
+    return;
+  }
+  const uint64_t dead = ~*b.live;
+  const int lane = threadIdx.x & 63;
+  // whole waves over 64 consecutive vertices (T divides 64); padding rows
+  // (v >= n) hold every source bit in V
+  const long long first = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+  for (long long base = first; base < b.npad; base += stride) {
+    const long long v = base + lane;
+    const bool in = v < b.npad;
+    const bool full = !in || (b.V[v] | dead) == ~0ull;
+    const unsigned long long m = __ballot(full);
+    const unsigned long long grp = (T == 64 ? ~0ull : ((1ull << T) - 1)) << (lane & ~(T - 1));
+    const bool row_sat = (m & grp) == grp;
+    // block rows without tiles: the level kernel steps over them, so their
+    // next-frontier words are cleared here (a source's seed bit included)
+    const long long I = in ? v / T : 0;
+    const bool empty = in && b.rowptr[I] == b.rowptr[I + 1];
+    if (in && (row_sat || empty)) b.Fn[v] = 0;
+    if (in && (lane & (T - 1)) == 0) {
+      b.sat[I] = row_sat ? 1 : 0;
+      if (row_sat || empty) static_cast<W *>(b.fnzn)[I] = (W)0;
+    }
+  }
+}
+
+// Pull levels, after the saturation pass: chunk c may skip its tiles when
+// every block row it touches (part[c] .. part[c+1]) is saturated; chunks over
+// more than 32 block rows are not checked. One thread per chunk.
+__global__ void bfs_chunk_skip_kernel(BfsState b) {
+  if (b.ctl[2] || b.ctl[3]) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.nchunks) return;
+  const int Ia = b.part[c];
+  const int Ib = c + 1 < b.nchunks ? b.part[c + 1] : b.nbr - 1;
+  bool all = Ib - Ia <= 32;
+  for (int I = Ia; all && I <= Ib; ++I) all = b.sat[I] != 0;
+  b.skip[c] = all ? 1 : 0;
 }
 
 // Pushes frontier row r of A^T's block row I over its tiles [k0, k1) (lanes
@@ -423,6 +517,7 @@ __global__ void bfs_push_finish_kernel(BfsState b) {
   const int lane = threadIdx.x & 63;
   bool any = false;
   unsigned long long cost = 0;
+  uint64_t lnew = 0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x; I < b.nbr; I += stride) {
     uint32_t bits = 0;
@@ -431,6 +526,7 @@ __global__ void bfs_push_finish_kernel(BfsState b) {
       const long long v = I * T + r;
       const uint64_t nb = b.Fn[v];
       if (!nb) continue;
+      lnew |= nb;
       b.V[v] |= nb;
       for (int k = 0; k < kPlanes; ++k)
         if ((depth >> k) & 1) b.P[k * b.npad + v] |= nb;
@@ -443,6 +539,7 @@ __global__ void bfs_push_finish_kernel(BfsState b) {
     }
   }
   push_finish(b, any, cost, lane, blockIdx.x);
+  publish_live(b, lnew, lane, blockIdx.x);
 }
 
 template <class V>
@@ -550,14 +647,14 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   const long long nwords = ceil_div(nbr, T) * T;
   const int wb = word_bytes(T);
   const long long nchunks = ceil_div(nblocks, CH);
-  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv, pt, fc, bign, bigl;
+  DeviceBuffer F0, F1, Vb, Pb, z0, z1, ctl, src, ck, cv, pt, fc, bign, bigl, lv, lacc, sat, skp;
   BGB_HIP_TRY(F0.alloc(8 * npad, s));
   BGB_HIP_TRY(F1.alloc(8 * npad, s));
   BGB_HIP_TRY(Vb.alloc(8 * npad, s));
   BGB_HIP_TRY(Pb.alloc(8 * npad * kPlanes, s));
   BGB_HIP_TRY(z0.alloc(wb * nwords + 16, s));
   BGB_HIP_TRY(z1.alloc(wb * nwords + 16, s));
-  BGB_HIP_TRY(ctl.alloc(16, s));
+  BGB_HIP_TRY(ctl.alloc(8 * sizeof(int), s));
   BGB_HIP_TRY(src.alloc(sizeof(int) * nsrc, s));
   BGB_HIP_TRY(ck.alloc(sizeof(int) * (nchunks + 1), s));
   BGB_HIP_TRY(cv.alloc(8 * (nchunks + 1) * T, s));
@@ -565,6 +662,10 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   BGB_HIP_TRY(fc.alloc(sizeof(unsigned long long) * kCounters, s));
   BGB_HIP_TRY(bign.alloc(sizeof(int), s));
   BGB_HIP_TRY(bigl.alloc(sizeof(int2) * (At ? kBigCap : 1), s));
+  BGB_HIP_TRY(lv.alloc(sizeof(uint64_t), s));
+  BGB_HIP_TRY(lacc.alloc(sizeof(unsigned long long) * kCounters, s));
+  BGB_HIP_TRY(sat.alloc(nbr + 64, s));
+  BGB_HIP_TRY(skp.alloc(nchunks + 64, s));
   if (nchunks > 0)  // chunk -> first block row, once for every level
     chunk_rows_kernel<<<nblk(nbr), 256, 0, s>>>(rowptr, nbr, CH, (int)nchunks, pt.as<int>());
   const uint64_t full = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
@@ -578,8 +679,11 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z0.as<uint8_t>(), wb * nwords + 16, 0);
   bfs_fill_kernel<uint8_t><<<nfill(wb * nwords + 16), 256, 0, s>>>(z1.as<uint8_t>(), wb * nwords + 16, 0);
   bfs_fill_kernel<unsigned long long><<<1, kCounters, 0, s>>>(fc.as<unsigned long long>(), kCounters, 0ull);
+  // every source starts live
+  bfs_fill_kernel<uint64_t><<<1, 64, 0, s>>>(lv.as<uint64_t>(), 1, full);
+  bfs_fill_kernel<unsigned long long><<<1, kCounters, 0, s>>>(lacc.as<unsigned long long>(), kCounters, 0ull);
   const bool push = At != nullptr;
-  const int init_ctl[4] = {1, 0, 0, push ? 1 : 0};  // depth 1: a push level (the frontier is the sources)
+  const int init_ctl[8] = {1, 0, 0, push ? 1 : 0, 0, 0, 0, 0};  // depth 1: a push level (the sources)
   BGB_HIP_TRY(hipMemcpyAsync(ctl.ptr, init_ctl, sizeof(init_ctl), hipMemcpyHostToDevice, s));
   BGB_HIP_TRY(hipMemcpyAsync(src.ptr, h_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, s));
   bfs_seed_kernel<T><<<1, 64, 0, s>>>(src.as<int>(), nsrc, F0.as<uint64_t>(), Vb.as<uint64_t>(), z0.ptr);
@@ -612,6 +716,10 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     b.colindT = colindT;
     b.big_n = bign.as<int>();
     b.big_list = bigl.as<int2>();
+    b.live = lv.as<uint64_t>();
+    b.live_acc = lacc.as<unsigned long long>();
+    b.sat = sat.as<uint8_t>();
+    b.skip = skp.as<uint8_t>();
   }
   const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
   const unsigned fblocks = nblk(ceil_div(nchunks, 64) * 64);  // one lane per chunk, whole waves
@@ -619,8 +727,9 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
   const long long zbytes = wb * nwords + 16;
   auto one_level = [&](int p) {
-    bfs_clear_kernel<<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
+    bfs_clear_kernel<T><<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
     if (nchunks > 0) {
+      bfs_chunk_skip_kernel<<<nblk(nchunks), 256, 0, s>>>(st[p]);
       bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
       bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
       if (push) {
@@ -629,7 +738,8 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
         bfs_push_finish_kernel<T><<<nblk(nbr), 256, 0, s>>>(st[p]);
       }
     }
-    bfs_advance_kernel<<<1, kCounters, 0, s>>>(ctl.as<int>(), fc.as<unsigned long long>(), push_limit);
+    bfs_advance_kernel<<<1, kCounters, 0, s>>>(ctl.as<int>(), fc.as<unsigned long long>(), push_limit,
+                                                lacc.as<unsigned long long>(), lv.as<uint64_t>());
   };
   auto level_pair = [&]() {
     for (int p = 0; p < 2; ++p) one_level(p);

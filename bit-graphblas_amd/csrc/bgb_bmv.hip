@@ -59,6 +59,7 @@ struct BmvArgs {
   int *carry_key;
   void *carry_val;
   int *part;  // part[c]: block row of chunk c's first tile (cached per stream and rowptr)
+  int fixup_blocks;  // workgroups of the fix-up pass proper; any after them zero the empty block rows
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -333,13 +334,9 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   I = active ? I : rw.Ia;
   const int I0 = I;
 
-  Acc acc[R], zero[R];
+  Acc acc[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) acc[i] = zero[i] = (Acc)0;
-
-  // Leading empty block rows end at position 0: the group holding tile 0 owns them.
-  if (active && pa == 0)
-    for (int r = 0; r < I0; ++r) flush_row<T, OP>(a, r, q, zero);
+  for (int i = 0; i < R; ++i) acc[i] = (Acc)0;
 
   int held = -1;
   Acc hv[R];
@@ -440,10 +437,24 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         ++I;
         e = e2;
         e2 = row_end(I + 1);
-        while (e == pos) {  // empty rows
-          flush_row<T, OP>(a, I, q, zero);
-          ++I;
-          e = e2;
+        if (e == pos) {
+          // Empty block rows (no tiles) follow: the walk steps over them (the
+          // fix-up pass zeroes them when C is overwritten) to the first row
+          // ending after pos -- the next row, or by a binary search over the
+          // row ends, so a long run of empty rows costs one lane log2 steps.
+          if (e2 > pos) {
+            ++I;
+            e = e2;
+          } else {
+            int lo = I + 2, up = ST ? (rw.Ib + 1 < a.nbr ? rw.Ib + 1 : a.nbr) : a.nbr;
+            while (lo < up) {
+              const int mid = (lo + up) >> 1;
+              if (row_end(mid) > pos) up = mid;
+              else lo = mid + 1;
+            }
+            I = lo;
+            e = row_end(I);
+          }
           e2 = row_end(I + 1);
         }
       }
@@ -669,10 +680,29 @@ template <int T, int OP>
 __global__ __launch_bounds__(256) void bmv_fixup_kernel(BmvArgs a) {
   using Acc = typename AccOf<OP>::type;
   using W = typename Tile<T>::W;
+  if ((int)blockIdx.x >= a.fixup_blocks) {
+    // Workgroups past the fix-up proper (launched when C is overwritten)
+    // zero the empty block rows, which the main pass steps over. No carry
+    // lands in an empty row, so these stores meet no other writer.
+    const long long stride = (long long)(gridDim.x - a.fixup_blocks) * blockDim.x;
+    for (long long I = (long long)(blockIdx.x - a.fixup_blocks) * blockDim.x + threadIdx.x; I < a.nbr;
+         I += stride) {
+      if (a.rowptr[I] != a.rowptr[I + 1]) continue;
+      if constexpr (OP == OP_BITS || OP == OP_MASKED) {
+        static_cast<W *>(a.C)[I] = (W)0;
+      } else {
+        typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+        U4 *y = reinterpret_cast<U4 *>(static_cast<uint32_t *>(a.C) + I * T);
+#pragma unroll
+        for (int r = 0; r < T / 4; ++r) y[r] = U4{0u, 0u, 0u, 0u};
+      }
+    }
+    return;
+  }
   if constexpr (OP == OP_BITS || OP == OP_MASKED) {
     // first word whose aligned dword reaches past C[nbr-1] (nbr: none)
     const int tail = sizeof(W) < 4 ? (int)((((long long)a.nbr * sizeof(W)) & ~3ll) / sizeof(W)) : a.nbr;
-    if (blockIdx.x == gridDim.x - 1) {  // the tail workgroup
+    if ((int)blockIdx.x == a.fixup_blocks - 1) {  // the tail workgroup
       if constexpr (sizeof(W) < 4)
         if (threadIdx.x < 64 && tail < a.nbr) bits_tail<T, OP>(a, tail, threadIdx.x);
       return;
@@ -950,7 +980,10 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   // per 64 chunks
   constexpr bool kBits = OP == OP_BITS || OP == OP_MASKED;
   const long long fix_blocks = kBits ? ceil_div(nchunks, 256) + 1 : ceil_div(ceil_div(nchunks, 64) * 64, 256);
-  bmv_fixup_kernel<T, OP><<<(unsigned)fix_blocks, 256, 0, s>>>(a);
+  a.fixup_blocks = (int)fix_blocks;
+  // overwriting calls also zero the empty block rows (accumulate leaves them)
+  const long long zero_blocks = (a.accumulate && OP != OP_MASKED) ? 0 : std::min<long long>(ceil_div(a.nbr, 256), 1024);
+  bmv_fixup_kernel<T, OP><<<(unsigned)(fix_blocks + zero_blocks), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }
@@ -1029,7 +1062,7 @@ int bmv_entry(int T, const void *A, const void *B, void *C, const int *rowptr, c
       BGB_HIP_TRY(hipGetLastError());
       return BGB_OK;
     }
-    BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr};
+    BmvArgs a{A, B, C, rowptr, colind, mask, nbr, nblocks, accumulate, 0, nullptr, nullptr, nullptr, 0};
     switch (T) {
       case 4: return launch_bmv<4, OP>(a, s, scr);
       case 8: return launch_bmv<8, OP>(a, s, scr);
