@@ -69,6 +69,10 @@ SIGNATURES = {
     "bgb_bfs_level_times": (_c_int, [_c_int, _vp, _vp, _vp]),
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "bgb_tile_profile_csr": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
+    "bgb_autotune_profile": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp]),
+    "bgb_autotune_measure": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                      _vp]),
     "bgb_autotune_tiledim": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp]),
     "bgb_autotune": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp]),
     "bgb_csr_from_coo": (_c_int, [_c_int, _c_int, _c_ll, _vp, _vp, _c_int, _c_int, ctypes.POINTER(_vp)]),

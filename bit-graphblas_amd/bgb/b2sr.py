@@ -215,6 +215,27 @@ def bfs_multi(A, sources, use_graph=True, push=None):
     return lev, nl.value
 
 
+def autotune_measure(csr, op="bin_bin", allowed=(4, 8, 16, 32), reps=10, device="cuda"):
+    """Measured tuning on the GPU (bgb_autotune_measure) for a host CSR:
+    {"T", "run_length", "x_staging", "times_us": {(T, S, placement): us}}."""
+    dev = torch.device(device)
+    d_rp = torch.from_numpy(csr.rowptr).to(dev)
+    d_ci = torch.from_numpy(csr.colind if csr.nnz else np.zeros(1, np.int32)).to(dev)
+    times = np.zeros(16, np.float64)
+    t, rl, xs = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    mask = sum(1 << i for i, T in enumerate((4, 8, 16, 32)) if T in allowed)
+    call("bgb_autotune_measure", {"bin_bin": 0, "bin_full_full": 1}[op], csr.nrows, csr.ncols, csr.nnz, _ptr(d_rp),
+         _ptr(d_ci), mask, reps, ctypes.byref(t), ctypes.byref(rl), ctypes.byref(xs), times.ctypes.data, _stream())
+    out = {}
+    for i, T in enumerate((4, 8, 16, 32)):
+        for si, S in enumerate((8, 16)):
+            for xi, place in enumerate(("memory", "lds")):
+                v = times[i * 4 + si * 2 + xi]
+                if v > 0:
+                    out[(T, S, place)] = float(v)
+    return {"T": t.value, "run_length": rl.value, "x_staging": xs.value, "times_us": out}
+
+
 def bfs_level_times(A, sources, push=None):
     """Runs bfs_multi with the eager level loop, one level at a time between HIP
     events: returns (levels, nlevels, [(ms, pulled)] per level)."""

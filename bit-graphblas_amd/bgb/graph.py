@@ -99,23 +99,58 @@ def tile_stats(csr):
 
 OP_FAMILY = {"bin_bin": 0, "bin_full_full": 1, "k16": 2, "bfs": 3}
 
+TILE_DIMS4 = (4, 8, 16, 32)
 
-def autotune(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):
-    """Per-graph kernel configuration: {"T", "run_length", "x_staging", "cost_us"}."""
-    nbd, _ = stats if stats is not None else tile_stats(csr)
-    nb = np.array([nbd[T] for T in (4, 8, 16, 32)], np.int64)
+
+class TileProfile(ctypes.Structure):
+    """bgb_tile_profile: per T = 4, 8, 16, 32 the tiles, non-empty block rows,
+    the longest block row and the non-zero tile row words."""
+    _fields_ = [("tiles", ctypes.c_longlong * 4), ("nonempty_rows", ctypes.c_longlong * 4),
+                ("max_row_tiles", ctypes.c_longlong * 4), ("row_words", ctypes.c_longlong * 4),
+                ("nnz", ctypes.c_longlong), ("nrows", ctypes.c_int), ("ncols", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: {T: int(getattr(self, f)[i]) for i, T in enumerate(TILE_DIMS4)}
+                for f in ("tiles", "nonempty_rows", "max_row_tiles", "row_words")}
+
+
+def tile_profile(csr):
+    """The autotuner's features of a host CSR (bgb_tile_profile_csr)."""
+    p = TileProfile()
+    call("bgb_tile_profile_csr", csr.nrows, csr.ncols, csr.rowptr.ctypes.data,
+         csr.colind.ctypes.data if csr.nnz else None, ctypes.byref(p))
+    return p
+
+
+def autotune_profile(csr, op="bin_bin", allowed=TILE_DIMS4, profile=None):
+    """Per-graph configuration from the tile profile: {"T", "run_length",
+    "x_staging", "cost_us"} (bgb_autotune_profile)."""
+    p = profile if profile is not None else tile_profile(csr)
     cost = np.zeros(4, np.float64)
-    mask = sum(1 << i for i, T in enumerate((4, 8, 16, 32)) if T in allowed)
+    mask = sum(1 << i for i, T in enumerate(TILE_DIMS4) if T in allowed)
     t, rl, xs = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-    call("bgb_autotune", OP_FAMILY[op], csr.nrows, nb.ctypes.data, mask, ctypes.byref(t), ctypes.byref(rl),
+    call("bgb_autotune_profile", OP_FAMILY[op], ctypes.byref(p), mask, ctypes.byref(t), ctypes.byref(rl),
          ctypes.byref(xs), cost.ctypes.data)
     return {"T": t.value, "run_length": rl.value, "x_staging": xs.value,
-            "cost_us": {T: float(cost[i]) for i, T in enumerate((4, 8, 16, 32))}}
+            "cost_us": {T: float(cost[i]) for i, T in enumerate(TILE_DIMS4)}}
 
 
-def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), stats=None):
-    """(best T, predicted microseconds per T) from the tile statistics and the cost model."""
-    nbd, _ = stats if stats is not None else tile_stats(csr)
+def autotune(csr, op="bin_bin", allowed=(4, 8, 16, 32), profile=None):
+    """Per-graph configuration from the tile profile: {"T", "run_length",
+    "x_staging", "cost_us"} (bgb_autotune_profile)."""
+    return autotune_profile(csr, op, allowed, profile)
+
+
+def autotune_tiledim(csr, op="bin_bin", allowed=(4, 8, 16, 32), profile=None):
+    """(best T, predicted microseconds per T) from the tile profile."""
+    c = autotune_profile(csr, op, allowed, profile)
+    return c["T"], c["cost_us"]
+
+
+def autotune_counts(csr, op="bin_bin", allowed=(4, 8, 16, 32)):
+    """The count-only form (bgb_autotune_tiledim: tiles per T, no row
+    statistics): (best T, predicted microseconds per T)."""
+    nbd, _ = tile_stats(csr)
     nb = np.array([nbd[T] for T in (4, 8, 16, 32)], np.int64)
     cost = np.zeros(4, np.float64)
     mask = sum(1 << i for i, T in enumerate((4, 8, 16, 32)) if T in allowed)

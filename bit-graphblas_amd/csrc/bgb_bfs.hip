@@ -243,13 +243,17 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
         e2 = row_end(I + 1);
         if (e == pos) {
           // block rows without tiles follow (their words were cleared by the
-          // level's first pass): step to the first row ending after pos, by a
-          // binary search over the row ends when more than one is empty
-          if (e2 > pos) {
-            ++I;
-            e = e2;
+          // level's first pass): step to the first row ending after pos, one
+          // row end per step in the LDS window (at most kWin rows), by a
+          // binary search over the row ends in memory
+          if constexpr (ST) {
+            do {
+              ++I;
+              e = e2;
+              e2 = row_end(I + 1);
+            } while (e == pos);
           } else {
-            int lo = I + 2, up = ST ? (rw.Ib + 1 < b.nbr ? rw.Ib + 1 : b.nbr) : b.nbr;
+            int lo = I + 1, up = b.nbr;
             while (lo < up) {
               const int mid = (lo + up) >> 1;
               if (row_end(mid) > pos) up = mid;
@@ -257,8 +261,8 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
             }
             I = lo;
             e = row_end(I);
+            e2 = row_end(I + 1);
           }
-          e2 = row_end(I + 1);
         }
         if (I < b.nbr) load_visited<T>(b, I, q, vr);
       }

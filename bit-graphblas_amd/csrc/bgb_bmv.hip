@@ -440,13 +440,17 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         if (e == pos) {
           // Empty block rows (no tiles) follow: the walk steps over them (the
           // fix-up pass zeroes them when C is overwritten) to the first row
-          // ending after pos -- the next row, or by a binary search over the
-          // row ends, so a long run of empty rows costs one lane log2 steps.
-          if (e2 > pos) {
-            ++I;
-            e = e2;
+          // ending after pos. In the LDS window (at most kWin rows) one row
+          // end per step; in memory, where a chunk can span any number of
+          // empty rows, a binary search over the row ends.
+          if constexpr (ST) {
+            do {
+              ++I;
+              e = e2;
+              e2 = row_end(I + 1);
+            } while (e == pos);
           } else {
-            int lo = I + 2, up = ST ? (rw.Ib + 1 < a.nbr ? rw.Ib + 1 : a.nbr) : a.nbr;
+            int lo = I + 1, up = a.nbr;
             while (lo < up) {
               const int mid = (lo + up) >> 1;
               if (row_end(mid) > pos) up = mid;
@@ -454,8 +458,8 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
             }
             I = lo;
             e = row_end(I);
+            e2 = row_end(I + 1);
           }
-          e2 = row_end(I + 1);
         }
       }
     }

@@ -36,6 +36,11 @@ int host_guard(F &&body) {
   }
 }
 
+// BMV tuning knobs per tile dim (index log2(T) - 2), set by
+// bgb_set_run_length / bgb_set_x_staging.
+extern int g_run_length[5];
+extern int g_x_staging[5];
+
 inline bool valid_tiledim(int T) { return T == 4 || T == 8 || T == 16 || T == 32 || T == 64; }
 inline int word_bytes(int T) {
   return T == 64 ? 8 : T == 32 ? 4 : (T == 16 ? 2 : (T == 4 || T == 8 ? 1 : 0));

@@ -65,9 +65,157 @@ double op_bytes(int op, int T, long long nrows, long long nblocks) {
   return tiles + rp + nbr * T * 16 * (2.0 + 4.0);                     // FP16 X + FP32 Y
 }
 
+
+// Tile profile of a host CSR at every T: tiles and non-empty block rows
+// (distinct block columns per block row), the longest block row, and the
+// non-zero tile row words (distinct block columns per CSR row: the sum of
+// printMatrixStats_T's nnz_tile_per_row, utility.cu:808-930, and
+// estimatevec's count, :955-975, for sorted rows).
+int tile_profile(int nrows, int ncols, const int *rp, const int *ci, bgb_tile_profile *p) {
+  p->nrows = nrows;
+  p->ncols = ncols;
+  p->nnz = rp[nrows];
+  for (int t = 0; t < 4; ++t) {
+    const int T = kTs[t];
+    const long long nbr = (nrows + T - 1) / T;
+    long long tiles = 0, nonempty = 0, most = 0, words = 0;
+#pragma omp parallel reduction(+ : tiles, nonempty, words) reduction(max : most)
+    {
+      std::vector<int> buf;
+#pragma omp for schedule(dynamic, 256)
+      for (long long I = 0; I < nbr; ++I) {
+        const int r0 = (int)(I * T), r1 = (int)std::min<long long>((I + 1) * T, nrows);
+        buf.clear();
+        for (int r = r0; r < r1; ++r) {
+          int last = -1;
+          for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int J = ci[k] / T;
+            buf.push_back(J);
+            words += J != last;  // rows are sorted: a new block column starts a word
+            last = J;
+          }
+        }
+        if (buf.empty()) continue;
+        std::sort(buf.begin(), buf.end());
+        const long long d = std::unique(buf.begin(), buf.end()) - buf.begin();
+        tiles += d;
+        ++nonempty;
+        most = std::max(most, d);
+      }
+    }
+    p->tiles[t] = tiles;
+    p->nonempty_rows[t] = nonempty;
+    p->max_row_tiles[t] = most;
+    p->row_words[t] = words;
+  }
+  return BGB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Profile-driven model. Each (op, T, run length) is predicted from the kernel
+// path it takes on this matrix -- the short-row kernel (T <= 8, no block row
+// over kShortRowMax tiles), the chunked walk with the packed x staged in LDS
+// or gathered from memory, the FP sums -- as
+//   max(bytes / bandwidth, c0 + a * tiles),
+// constants fitted to MI355X runs of these kernels on R-MAT, Erdos-Renyi and
+// grid graphs (us, tiles in millions).
+constexpr long long kShortRowMax = 32;
+struct PathFit {
+  double c0;
+  double per_mtile[4];
+};
+const PathFit kBitShort = {5.0, {1.46, 2.2, 0, 0}};
+const PathFit kBitLds = {15.0, {3.2, 3.6, 9.0, 0}};
+const PathFit kBitMem = {15.0, {4.3, 4.6, 9.3, 24.0}};
+const PathFit kFpShort = {4.0, {3.5, 4.4, 0, 0}};
+const PathFit kFpChunk = {12.0, {6.0, 7.7, 27.0, 52.0}};
+
+bool lds_path(int T, const bgb_tile_profile &p, int cus = 256) {
+  const int t = T == 4 ? 0 : T == 8 ? 1 : T == 16 ? 2 : 3;
+  if (T > 16 || lds_x_bytes(T, p.nrows) > kLdsForX) return false;
+  const double wb = T == 16 ? 2 : 1;
+  return (double)p.tiles[t] * (4.0 + T * wb) >= 2.0 * (double)lds_x_bytes(T, p.nrows) * cus;
+}
+
+// Predicted microseconds of one call of op at tile dim index t, run length S.
+double predict_us(int op, int t, int S, const bgb_tile_profile &p, int *x_staging) {
+  const int T = kTs[t];
+  const double mt = (double)p.tiles[t] / 1e6;
+  const long long nbr = (p.nrows + T - 1) / T;
+  const double mean = (double)p.tiles[t] / (double)std::max<long long>(p.nonempty_rows[t], 1);
+  const bool uniform = (double)p.max_row_tiles[t] <= 4.0 * mean;
+  const bool short_rows = T <= 8 && p.max_row_tiles[t] <= kShortRowMax;
+  if (x_staging) *x_staging = 1;
+  double issue;
+  if (op == 0 || op == 1) {
+    const PathFit *f;
+    if (short_rows) f = op == 0 ? &kBitShort : &kFpShort;
+    else if (op == 0 && lds_path(T, p)) {
+      f = &kBitLds;
+      if (x_staging) *x_staging = 2;
+    } else f = op == 0 ? &kBitMem : &kFpChunk;
+    double a = f->per_mtile[t];
+    // FP x segments gathered from beyond the L2 with no reuse (uniform
+    // columns, x larger than the L2s hold): measured 2.0-2.4x the per-tile cost
+    if (op == 1 && !short_rows && T <= 8 && uniform && p.nrows >= (2 << 20)) a *= 2.2;
+    issue = f->c0 + a * mt;
+    if (!short_rows && S == 16) {
+      // runs of 16 tiles: slower at T <= 8 (register budget), faster at T = 16
+      // for long uniform rows and at T = 32 for large matrices with rows of
+      // more than a few tiles (fewer chunks and carries)
+      const bool big = p.tiles[t] >= 5000000;
+      const double f16 = T <= 8 ? 1.25 : T == 16 ? ((big && uniform && mean >= 64) ? 0.88 : 1.1)
+                                                 : ((big && mean >= 16) ? 0.95 : 1.05);
+      issue *= f16;
+    }
+  } else {
+    const Model &m = kModels[op];
+    issue = ((double)p.tiles[t] * m.tile_ns[t] + (double)nbr * m.row_ns[t]) / 1e3 + 3.0;
+  }
+  const double bw = op == 0 ? 5500.0 : 5000.0;
+  const double mem_us = op_bytes(op, T, p.nrows, p.tiles[t]) / (bw * 1e3);
+  return std::max(issue, mem_us);
+}
+
 }  // namespace
 
 extern "C" {
+
+int bgb_autotune_profile(int op, const bgb_tile_profile *prof, int allowed_mask, int *T_out, int *run_length_out,
+                         int *x_staging_out, double *cost4_us) {
+  return bgb::host_guard([&]() -> int {
+    bgb::clear_error();
+    BGB_REQUIRE(op >= 0 && op <= 3 && prof && T_out, BGB_ERR_ARG, "bad arguments");
+    if (op == 2) allowed_mask &= 0xC;  // the MFMA path takes T = 16, 32
+    BGB_REQUIRE(allowed_mask & 0xF, BGB_ERR_ARG, "no tile dim allowed");
+    double best = 1e300;
+    for (int t = 0; t < 4; ++t) {
+      int xs8 = 1, xs16 = 1;
+      const double c8 = predict_us(op, t, 8, *prof, &xs8);
+      // run lengths are a knob of the bit and FP BMV kernels only
+      const double c16 = op <= 1 ? predict_us(op, t, 16, *prof, &xs16) : 1e300;
+      const double c = std::min(c8, c16);
+      if (cost4_us) cost4_us[t] = c;
+      if (((allowed_mask >> t) & 1) && c < best) {
+        best = c;
+        *T_out = kTs[t];
+        if (run_length_out) *run_length_out = c16 < c8 ? 16 : 8;
+        if (x_staging_out) *x_staging_out = c16 < c8 ? xs16 : xs8;
+      }
+    }
+    return BGB_OK;
+  });
+}
+
+
+int bgb_tile_profile_csr(int nrows, int ncols, const int *h_rowptr, const int *h_colind, bgb_tile_profile *out) {
+  return bgb::host_guard([&]() -> int {
+    bgb::clear_error();
+    BGB_REQUIRE(nrows >= 0 && ncols >= 0 && h_rowptr && out, BGB_ERR_ARG, "bad arguments");
+    BGB_REQUIRE(h_rowptr[nrows] == 0 || h_colind, BGB_ERR_ARG, "null colind");
+    return tile_profile(nrows, ncols, h_rowptr, h_colind, out);
+  });
+}
 
 int bgb_tile_stats(int nrows, int ncols, const int *h_rowptr, const int *h_colind, long long *h_nblocks4,
                    long long *h_nonempty4) {

@@ -342,6 +342,51 @@ int bgb_tile_row_stats(int T, const void *d_A, const int *d_rowptr, int nblockro
 int bgb_estimate_tiles(int T, int nrows, const int *d_rowptr, const int *d_colind,
                        long long *d_out, bgb_stream_t stream);
 
+/* Tile profile of a host CSR for T = 4, 8, 16, 32 (index 0..3): the
+ * quantities the reference's printMatrixStats_T and estimatevec report
+ * (utility.cu:808-975), summed over the matrix, and the longest block row.
+ * These are the autotuner's features. */
+typedef struct {
+  long long tiles[4];          /* B2SR tiles (nblocks) */
+  long long nonempty_rows[4];  /* block rows holding at least one tile */
+  long long max_row_tiles[4];  /* most tiles in one block row */
+  long long row_words[4];      /* non-zero tile row words: sum of nnz_tile_per_row
+                                  = estimatevec's count for sorted rows */
+  long long nnz;
+  int nrows, ncols;
+} bgb_tile_profile;
+
+int bgb_tile_profile_csr(int nrows, int ncols, const int *h_rowptr, const int *h_colind,
+                         bgb_tile_profile *out);
+
+/* The kernel configuration the library would run for an op family (0:
+ * bin_bin_*, 1: bin_full_full, 2: K=16 MFMA, 3: a pull level of the batched
+ * BFS) on a matrix with this profile: T (allowed_mask bit t enables T = 4 << t)
+ * with the lowest predicted time, the run length per lane group and the
+ * packed-x placement (1 = gathered from memory, 2 = staged in LDS). The model
+ * predicts each T from the kernel path it would take -- short-row kernel
+ * (T <= 8 and no block row over 32 tiles), LDS-staged or gathered x, FP
+ * set-bit walk -- as max(bytes / bandwidth, per-tile + per-row-word +
+ * per-block-row issue cost), constants fitted to MI355X runs. cost4_us
+ * (may be NULL) receives the predictions. */
+int bgb_autotune_profile(int op, const bgb_tile_profile *prof, int allowed_mask, int *T_out,
+                         int *run_length_out, int *x_staging_out, double *cost4_us);
+
+/* Measured tuning (no reference counterpart): packs the device CSR at every
+ * allowed T (allowed_mask bit t: T = 4 << t) into temporary buffers and times
+ * the op's BMV -- op 0: bin_bin_full (int32) with x Bernoulli(0.05), op 1:
+ * bin_full_full with x uniform in [0, 1) -- for run lengths 8 and 16 and, for
+ * op 0 at T <= 16, x gathered from memory and staged in LDS, with HIP events
+ * over `reps` calls after a warm-up call on `stream`. Returns the fastest
+ * configuration; h_times_us[16] (may be NULL) receives each time in
+ * microseconds at [t*4 + s*2 + x] (t: T index, s: run length 8 / 16, x:
+ * gathered / LDS), 0 where not run. Sets the process-wide knobs while it
+ * runs and restores them; synchronises. */
+int bgb_autotune_measure(int op, int nrows, int ncols, int nnz, const int *d_csrRowPtr,
+                         const int *d_csrColInd, int allowed_mask, int reps, int *T_out,
+                         int *run_length_out, int *x_staging_out, double *h_times_us,
+                         bgb_stream_t stream);
+
 /* Picks T for an op family (0: bin_bin_*, 1: bin_full_full, 2: K=16 MFMA,
  * 3: a level of the batched BFS)
  * from the tile counts with a cost model fitted to MI355X runs of these

@@ -149,13 +149,42 @@ def test_autotuner_prefers_small_tiles_on_sparse_random_graphs():
     assert T in (4, 8) and cost[T] == min(cost.values())
     T16, _ = graph.autotune_tiledim(g, "k16")
     assert T16 in (16, 32)
-    # a dense block structure favours big tiles: 32x32 all-ones blocks on the diagonal
-    ii, jj = np.meshgrid(np.arange(32), np.arange(32))
-    src = np.concatenate([ii.ravel() + 32 * b for b in range(200)])
-    dst = np.concatenate([jj.ravel() + 32 * b for b in range(200)])
-    d = graph.from_coo(6400, 6400, src, dst, False, False)
-    Td, _ = graph.autotune_tiledim(d, "bin_bin")
-    assert Td == 32
+    # the count-only form agrees on the small-tile choice
+    assert graph.autotune_counts(g, "bin_bin")[0] in (4, 8)
+
+
+def test_tile_profile_matches_the_oracle_statistics(orc):
+    """Tiles and non-empty block rows per T equal the oracle's B2SR; the row
+    words equal estimatevec's count; the longest block row is the largest
+    rowptr difference."""
+    g = graph.rmat(12, 8, seed=4)
+    p = graph.tile_profile(g).as_dict()
+    for T in (4, 8, 16, 32):
+        brp, _, _ = orc.b2sr(T, g.nrows, g.ncols, g.rowptr, g.colind)
+        assert p["tiles"][T] == brp[-1]
+        assert p["nonempty_rows"][T] == int(np.count_nonzero(np.diff(brp)))
+        assert p["max_row_tiles"][T] == int(np.diff(brp).max())
+        assert p["row_words"][T] == orc.estimate_tiles(g.rowptr, g.colind, T)
+
+
+def test_autotuner_model_paths():
+    """A 2-D grid (5 tiles per block row at T <= 8) takes the short-row kernel
+    at T = 8; one million dense 32 x 32 blocks (a profile built by hand: tiles
+    hold 4 x the bytes per edge at T = 4 than at T = 32) favour T = 32."""
+    c = graph.autotune(graph.grid2d(1024, 1024), "bin_bin")
+    assert c["T"] in (4, 8) and c["x_staging"] == 1
+    p = graph.TileProfile()
+    nb = 1 << 20
+    p.nrows = p.ncols = 32 * nb
+    p.nnz = 1024 * nb
+    for i, T in enumerate((4, 8, 16, 32)):
+        k = 32 // T
+        p.tiles[i] = nb * k * k
+        p.nonempty_rows[i] = nb * k
+        p.max_row_tiles[i] = k
+        p.row_words[i] = nb * k * k * T
+    c = graph.autotune_profile(None, "bin_bin", profile=p)
+    assert c["T"] == 32, c
 
 
 def test_tuning_knobs_validate_and_round_trip():
