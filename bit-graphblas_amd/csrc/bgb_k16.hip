@@ -108,8 +108,8 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
   float4_t D[MH], Z[MH];
 #pragma unroll
   for (int mh = 0; mh < MH; ++mh) D[mh] = Z[mh] = float4_t{0.f, 0.f, 0.f, 0.f};
-  if (k0 == 0)
-    for (int r = 0; r < I; ++r) k16_flush<T>(G, r, lane, Z);
+  // (block rows without tiles, before row I or anywhere in the walk, are left
+  // to k16_zero_empty_kernel)
   // e: end of row I; e2: end of row I + 1 (prefetched, read at the next row end)
   int e = sload(G.rowptr, I + 1);
   int e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
@@ -168,10 +168,24 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
         ++I;
         e = e2;
         e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
-        while (e == pos) {  // empty rows
-          k16_flush<T>(G, I, lane, Z);
-          ++I;
-          e = e2;
+        if (e == pos) {
+          // block rows without tiles follow: gallop over the row ends (1, 2,
+          // 4, ... rows ahead), then bisect, to the first row ending after pos
+          auto row_end = [&](int r) { return r < G.nbr ? sload(G.rowptr, r + 1) : INT_MAX; };
+          int lo = I + 1, step = 1, hi = I + 1;
+          while (hi < G.nbr && row_end(hi) <= pos) {
+            lo = hi + 1;
+            step *= 2;
+            hi = I + step;
+          }
+          if (hi > G.nbr) hi = G.nbr;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (row_end(mid) > pos) hi = mid;
+            else lo = mid + 1;
+          }
+          I = __builtin_amdgcn_readfirstlane(lo);
+          e = row_end(I);
           e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
         }
       }
@@ -275,6 +289,42 @@ __global__ __launch_bounds__(256) void k16_fixup_kernel(const int *carry_key, co
       }
       flush((first && cg == prev_g && ck == prev_k) || (cg == next_g && ck == next_k));
     }
+  }
+}
+
+// Zeroes the output rows of block rows without tiles (the walk steps over
+// them): one thread per 16 consecutive outputs of a block row, over the
+// block rows of every graph (row_start: prefix of the graphs' block-row
+// counts; graphs == nullptr for one graph, `single`).
+__global__ __launch_bounds__(256) void k16_zero_empty_kernel(const K16Graph *graphs, const long long *row_start,
+                                                             const int *tdim, int ngraphs, K16Graph single,
+                                                             int single_T) {
+  const long long total = graphs ? row_start[ngraphs] * 32 : (long long)single.nbr * single_T;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    K16Graph G = single;
+    int T = single_T;
+    long long r = t;  // (block row, output row) index within the graph
+    if (graphs) {
+      // 32 slots per block row whatever T (T = 16 uses the first 16)
+      const long long br = t / 32;
+      int lo = 0, hi = ngraphs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_start[mid] <= br) lo = mid;
+        else hi = mid - 1;
+      }
+      G = graphs[lo];
+      T = tdim[lo];
+      const int j = (int)(t % 32);
+      if (j >= T) continue;
+      r = (br - row_start[lo]) * T + j;
+    }
+    const long long I = r / T;
+    if (G.rowptr[I] != G.rowptr[I + 1]) continue;
+    float4_t *y = reinterpret_cast<float4_t *>(G.y + r * kK);
+#pragma unroll
+    for (int k = 0; k < kK / 4; ++k) y[k] = float4_t{0.f, 0.f, 0.f, 0.f};
   }
 }
 
@@ -414,6 +464,8 @@ int bgb_bmv_bin_full_full_k16(int T, const void *d_A, const void *d_Xb, float *d
     if (T == 16) k16_kernel<16><<<blocks, 64 * kWaves, 0, s>>>(a);
     else k16_kernel<32><<<blocks, 64 * kWaves, 0, s>>>(a);
     BGB_HIP_TRY(hipGetLastError());
+    if (!accumulate)
+      k16_zero_empty_kernel<<<nb256((long long)nblockrows * T), 256, 0, s>>>(nullptr, nullptr, nullptr, 0, a.g, T);
     k16_fixup_kernel<<<nb256(ceil_div(nchunks, kFixChunks) * 64), 256, 0, s>>>(a.carry_key, a.carry_val, nchunks, nullptr, nullptr,
                                                              nullptr, a.g, T);
     BGB_HIP_TRY(hipGetLastError());
@@ -429,7 +481,7 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::vector<K16Graph> gs(ngraphs);
     std::vector<int> td(ngraphs);
-    std::vector<long long> cs(ngraphs + 1, 0);
+    std::vector<long long> cs(ngraphs + 1, 0), rs(ngraphs + 1, 0);
     // T = 16 graphs first, then T = 32: one contiguous chunk range per tile dim
     std::vector<int> order;
     order.reserve(ngraphs);
@@ -451,14 +503,16 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
                           d_Y + h.y_off, nbr, h.nblocks, 0};
       td[slot] = h.T;
       cs[slot + 1] = cs[slot] + (h.nblocks > 0 ? ceil_div(h.nblocks, kChunk) : 0);
+      rs[slot + 1] = rs[slot] + nbr;
       if (h.nblocks == 0)
         k16_fill_kernel<float><<<nb256((long long)nbr * h.T * kK), 256, 0, s>>>(d_Y + h.y_off, (long long)nbr * h.T * kK, 0.f);
     }
     const long long total = cs[ngraphs];
-    DeviceBuffer dg, dt, dc, q, ck, cv, go;
+    DeviceBuffer dg, dt, dc, dr, q, ck, cv, go;
     BGB_HIP_TRY(dg.alloc(sizeof(K16Graph) * ngraphs, s));
     BGB_HIP_TRY(dt.alloc(sizeof(int) * ngraphs, s));
     BGB_HIP_TRY(dc.alloc(sizeof(long long) * (ngraphs + 1), s));
+    BGB_HIP_TRY(dr.alloc(sizeof(long long) * (ngraphs + 1), s));
     BGB_HIP_TRY(q.alloc(2 * sizeof(unsigned long long), s));  // one queue per tile dim
     BGB_HIP_TRY(ck.alloc(sizeof(int) * (total + 1), s));
     BGB_HIP_TRY(cv.alloc(sizeof(float) * 32 * kK * (total + 1), s));
@@ -466,6 +520,10 @@ int bgb_k16_batched(int ngraphs, const bgb_k16_graph *h_graphs, const int *d_row
     BGB_HIP_TRY(hipMemcpyAsync(dg.ptr, gs.data(), sizeof(K16Graph) * ngraphs, hipMemcpyHostToDevice, s));
     BGB_HIP_TRY(hipMemcpyAsync(dt.ptr, td.data(), sizeof(int) * ngraphs, hipMemcpyHostToDevice, s));
     BGB_HIP_TRY(hipMemcpyAsync(dc.ptr, cs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
+    BGB_HIP_TRY(hipMemcpyAsync(dr.ptr, rs.data(), sizeof(long long) * (ngraphs + 1), hipMemcpyHostToDevice, s));
+    // the walk steps over block rows without tiles: zero their outputs
+    k16_zero_empty_kernel<<<nb256(rs[ngraphs] * 32), 256, 0, s>>>(dg.as<K16Graph>(), dr.as<long long>(), dt.as<int>(),
+                                                                 ngraphs, K16Graph{}, 0);
     k16_fill_kernel<unsigned long long><<<1, 64, 0, s>>>(q.as<unsigned long long>(), 2, 0ull);
     BGB_HIP_TRY(hipGetLastError());
     if (total > 0) {

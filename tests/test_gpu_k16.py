@@ -61,6 +61,32 @@ def test_k16_single(orc, dev, T):
         np.testing.assert_allclose(Y2.cpu().numpy(), 2 * Y.cpu().numpy(), rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("T", (16, 32))
+def test_k16_empty_block_rows(orc, dev, T):
+    """Block rows without tiles -- leading, scattered and a long trailing run
+    (200 000 vertices, edges only among 20 000 of them) -- are zero in the
+    output (which starts dirty), single graph and batched."""
+    import torch
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(60 + T)
+    n = 200000
+    src = rng.integers(1000, 21000, 150000)
+    dst = rng.integers(1000, 21000, 150000)
+    keep = ((src // 64) % 3 != 0) & ((dst // 64) % 3 != 0)  # isolated stretches of 64 vertices
+    csr = graph.from_coo(n, n, src[keep], dst[keep], True, True)
+    A = B.B2SR.from_csr(csr, T, dev)
+    X, X64 = rhs(n, 70 + T, dev)
+    Xb = B.k16_pack_rhs(X, A.nblockrows * T)
+    Y = B.bmv_k16(A, Xb, out=dirty(A.nblockrows * T * 16, torch.float32, dev))
+    check(orc, csr, Y, X64, f"empty block rows T={T}")
+    batch = B.K16Batch([A, A], [Xb, Xb])
+    batch.y.view(torch.uint8).fill_(0xFF)
+    batch.run()
+    for i in range(2):
+        check(orc, csr, batch.result(i), X64, f"empty block rows batched #{i} T={T}")
+
+
 @pytest.mark.timeout(120)
 def test_k16_persistent_batch(orc, dev):
     """Graphs of uneven sizes (log-uniform n), half ER, half R-MAT, tile dim from
