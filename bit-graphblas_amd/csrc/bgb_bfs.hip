@@ -68,7 +68,8 @@ struct BfsState {
   uint64_t *P;           // kPlanes bit-planes, stride npad
   long long npad;        // NBR * T
   uint64_t full;         // all source bits
-  int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done, [3] push level
+  int *ctl;              // [0] depth of this level, [1] any-new flag, [2] done, [3] push level,
+                         // [5] some chunk of this pull level is skipped
   unsigned long long *fcnt;  // kCounters counters of the next frontier's push cost (tiles)
   int *carry_key;
   uint64_t *carry_val;
@@ -94,6 +95,8 @@ __device__ __forceinline__ void publish_live(const BfsState &b, uint64_t ln, int
 }
 
 // Records the vertices first reached at this level (new bits) for one row.
+// The level planes are OR-ed with atomics whose result is not used: a plain
+// read-modify-write would make the wave wait for the read.
 __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint64_t vold, uint64_t nb,
                                            int depth) {
   b.Fn[v] = nb;
@@ -101,16 +104,18 @@ __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint6
     b.V[v] = vold | nb;
 #pragma unroll
     for (int k = 0; k < kPlanes; ++k)
-      if ((depth >> k) & 1) b.P[k * b.npad + v] |= nb;
+      if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), nb);
     return true;
   }
   return false;
 }
 
-template <int T>
+// New bits of block row I (lanes q of its tile group): acc & ~visited.
+// `row_len(I)` is the row's tile count in A^T (its push cost per vertex).
+template <int T, class RL>
 __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const uint64_t (&acc)[Tile<T>::R],
                                           const uint64_t (&vr)[Tile<T>::R], int depth, bool &any,
-                                          unsigned long long &cost, uint64_t &lnew) {
+                                          unsigned long long &cost, uint64_t &lnew, RL &&row_len) {
   constexpr int R = Tile<T>::R, L = Tile<T>::L;
   using W = typename Tile<T>::W;
   uint32_t bits = 0;
@@ -125,30 +130,50 @@ __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const
     }
   }
   // push cost of the rows added: one walk of A^T's block row I each
-  if (b.rowptrT && bits) cost += (unsigned long long)__popc(bits) * (b.rowptrT[I + 1] - b.rowptrT[I]);
+  if (b.rowptrT && bits) cost += (unsigned long long)__popc(bits) * row_len(I);
 #pragma unroll
   for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
   if (q == 0) static_cast<W *>(b.fnzn)[I] = (W)bits;
 }
 
+// Row words of a tile restricted to the columns whose frontier word is
+// non-empty (x: the packed frontier-non-empty word of the tile's block column,
+// bit T-1-c for column c), in place: every row slot of the tile word ANDed
+// with x.
 template <int T>
-__device__ __forceinline__ void load_visited(const BfsState &b, int I, int q, uint64_t (&vr)[Tile<T>::R]) {
-  constexpr int R = Tile<T>::R;
+__device__ __forceinline__ void mask_cols(uint32_t (&F)[Tile<T>::NW], uint32_t x) {
+  constexpr int NW = Tile<T>::NW;
+  const uint32_t rep = T == 4 || T == 8 ? x * 0x01010101u : (T == 16 ? x * 0x00010001u : x);
 #pragma unroll
-  for (int i = 0; i < R; ++i) vr[i] = b.V[(long long)I * T + q * R + i];
+  for (int j = 0; j < NW; ++j) F[j] &= rep;
 }
+
+// Visited words of a chunk's vertices staged in LDS, per wave: the walk reads
+// one row's words at every row change, and from memory each read would be a
+// round trip the next tile waits for.
+constexpr int kVisStage = 512;
 
 template <int T, int S>
 __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
   __shared__ int s_win[kBfsWaves][kWinAlloc];
+  __shared__ uint64_t s_vis[kBfsWaves][kVisStage];
   if (b.ctl[2] || b.ctl[3]) return;  // BFS finished, or a push level: no-op
   const int depth = b.ctl[0];
   const int lane = threadIdx.x & 63;
-  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves +
-                                                   (threadIdx.x >> 6));
+  const int wid = threadIdx.x >> 6;
+  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves + wid);
   if (chunk >= b.nchunks) return;
+  // Every block row of the chunk saturated (bfs_chunk_skip_kernel): nothing
+  // can be new here and the rows' next-frontier words are already cleared.
+  // Tested before the tiles are loaded, so a skipped chunk reads none of
+  // them -- at levels where some chunk is skipped (ctl[5]; the others load
+  // their tiles at once).
+  if (b.ctl[5] && b.skip[chunk]) {
+    if (lane == 0) b.carry_key[chunk] = -1;
+    return;
+  }
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
@@ -159,34 +184,60 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   int J[S];
   const int n = (int)(pb - pa);
   if (active) load_run<T, S>(b.A, b.colind, pa, n, q, F, J);
-  // Every block row of the chunk saturated (bfs_chunk_skip_kernel): nothing
-  // can be new here and the rows' next-frontier words are already cleared.
-  // The flag is a scalar load: the test waits for it, not for the tiles
-  // already in flight (dropped when the chunk stops here).
-  if (b.skip[chunk]) {
-    if (lane == 0) b.carry_key[chunk] = -1;
-    return;
-  }
   const int Ia = b.part[chunk];
   const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
   const uint64_t dead = ~*b.live;  // sources whose BFS has no frontier left
   RowWindow rw;
   int wv[kWin / 64];
-  // frontier-non-empty words of the run's tiles, gathered unconditionally
-  // (block column 0 for missing tiles, then zeroed) so the loads issue together
-  uint32_t X[S];
-  auto gather_x = [&]() {
+  // Only columns with a non-empty frontier word are gathered: the tile words
+  // are masked by the packed frontier-non-empty words of their block columns
+  // (gathered unconditionally -- block column 0 for missing tiles, whose
+  // words are then cleared -- so that the loads issue together).
+  {
+    uint32_t X[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) X[s] = (uint32_t) static_cast<const W *>(b.fnz)[(active && s < n) ? J[s] : 0];
 #pragma unroll
-    for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : 0u;
-  };
-  gather_x();
+    for (int s = 0; s < S; ++s) mask_cols<T>(F[s], (active && s < n) ? X[s] : 0u);
+  }
   RowWindow::load(rowptr, Ia, Ib, lane, wv);
-  rw.put(rowptr, s_win[threadIdx.x >> 6], Ia, Ib, lane, wv);
-  // the walk, compiled for row ends in the LDS window and in memory
-  rw.dispatch([&](auto st) {
-  constexpr bool ST = decltype(st)::value;
+  // visited words of vertices [Ia*T, (Ib+1)*T), when they fit the wave's
+  // stage (and the row window is staged): unconditional loads, indices
+  // clamped into the range
+  const long long vbase = (long long)Ia * T;
+  const int vcount = (Ib - Ia + 1) * T;
+  const bool vst = vcount <= kVisStage && Ib + 2 - Ia <= kWin;  // wave-uniform
+  uint64_t *sv = s_vis[wid];
+  if (vst) {
+    uint64_t vt[kVisStage / 64];
+#pragma unroll
+    for (int j = 0; j < kVisStage / 64; ++j) {
+      const int i = lane + 64 * j;
+      vt[j] = b.V[vbase + (i < vcount ? i : vcount - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < kVisStage / 64; ++j) sv[lane + 64 * j] = vt[j];
+  }
+  rw.put(rowptr, s_win[wid], Ia, Ib, lane, wv);  // (its wave barrier also covers the stage)
+  const bool sym = b.rowptrT == rowptr;  // A^T's block rows are A's: lengths from the window
+  // The walk, compiled for: row ends and visited words in LDS (the common
+  // case), row ends in LDS and visited words in memory, both in memory.
+  auto walk = [&](auto st, auto vs) {
+  constexpr bool ST = decltype(st)::value, VS = decltype(vs)::value;
+  auto load_vis = [&](int I, uint64_t(&vr)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const long long v = (long long)I * T + q * R + i;
+      if constexpr (VS) vr[i] = sv[v - vbase];
+      else vr[i] = b.V[v];
+    }
+  };
+  auto row_len = [&](int I) -> int {
+    if constexpr (ST) {
+      if (sym) return rw.at_t<true>(I + 1) - rw.at_t<true>(I);
+    }
+    return b.rowptrT[I + 1] - b.rowptrT[I];
+  };
   int I;
   if constexpr (ST) I = window_group_rows<L, S>(rw, k0, lane, wv);
   else I = rw.find_t<false>(pa);
@@ -196,10 +247,10 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   bool any = false;
   unsigned long long nnew = 0;  // push cost of the rows this wave adds
   uint64_t lnew = 0;            // sources with a new bit in this wave's rows
-  uint64_t acc[R], vr[R], hv[R], hvr[R];
+  uint64_t acc[R], vr[R], hv[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) acc[i] = hv[i] = hvr[i] = 0;
-  if (active) load_visited<T>(b, I, q, vr);
+  for (int i = 0; i < R; ++i) acc[i] = hv[i] = 0;
+  if (active) load_vis(I, vr);
   int held = -1;
   // end of row I and, read one row ahead, of row I+1 (indices clamped into the
   // window / rowptr; clamped ends belong to rows past the chunk)
@@ -213,7 +264,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
       const uint64_t *__restrict__ fr = b.F + (long long)J[s] * T;
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        uint32_t w = row_word<T>(F[s], i) & X[s];
+        uint32_t w = row_word<T>(F[s], i);
         // Stop once the partial covers every live source that has not
         // reached the vertex yet: frontier words only hold live sources, so
         // more ORs cannot change acc & ~visited (this includes vertices every
@@ -227,14 +278,11 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
       const int pos = (int)(pa + s) + 1;
       if (pos == e) {
         if (held < 0) {
-          held = I;
+          held = I;  // its visited words are read again at the end
 #pragma unroll
-          for (int i = 0; i < R; ++i) {
-            hv[i] = acc[i];
-            hvr[i] = vr[i];
-          }
+          for (int i = 0; i < R; ++i) hv[i] = acc[i];
         } else {
-          bfs_flush<T>(b, I, q, acc, vr, depth, any, nnew, lnew);
+          bfs_flush<T>(b, I, q, acc, vr, depth, any, nnew, lnew, row_len);
         }
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0;
@@ -264,7 +312,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
             e2 = row_end(I + 1);
           }
         }
-        if (I < b.nbr) load_visited<T>(b, I, q, vr);
+        if (I < b.nbr) load_vis(I, vr);
       }
     }
   }
@@ -279,7 +327,11 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
     const uint64_t up = group_up<L, uint64_t>(sc[i]);
     if (g > 0 && kin == I0) hv[i] |= up;
   }
-  if (held >= 0) bfs_flush<T>(b, held, q, hv, hvr, depth, any, nnew, lnew);
+  if (held >= 0) {
+    uint64_t hvr[R];
+    load_vis(held, hvr);
+    bfs_flush<T>(b, held, q, hv, hvr, depth, any, nnew, lnew, row_len);
+  }
   const int last = (int)lmin2(G - 1, (b.nblocks - 1 - k0) / S);
   if (g == last) {
     const bool open = key < b.nbr;
@@ -294,7 +346,10 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) nnew += __shfl_xor(nnew, m);
   if (lane == 0 && nnew) atomicAdd(b.fcnt + (chunk & (kCounters - 1)), nnew);
-  });
+  };
+  if (!__builtin_amdgcn_readfirstlane((int)rw.staged)) walk(std::false_type{}, std::false_type{});
+  else if (__builtin_amdgcn_readfirstlane((int)vst)) walk(std::true_type{}, std::true_type{});
+  else walk(std::true_type{}, std::false_type{});
 }
 
 // Folds the carries of rows that cross chunks into this level's results. One
@@ -388,6 +443,7 @@ __global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < zbytes; i += stride) z[i] = 0;
     return;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.ctl[5] = 0;  // raised by bfs_chunk_skip_kernel
   const uint64_t dead = ~*b.live;
   const int lane = threadIdx.x & 63;
   // whole waves over 64 consecutive vertices (T divides 64); padding rows
@@ -418,12 +474,15 @@ __global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
 __global__ void bfs_chunk_skip_kernel(BfsState b) {
   if (b.ctl[2] || b.ctl[3]) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= b.nchunks) return;
-  const int Ia = b.part[c];
-  const int Ib = c + 1 < b.nchunks ? b.part[c + 1] : b.nbr - 1;
-  bool all = Ib - Ia <= 32;
-  for (int I = Ia; all && I <= Ib; ++I) all = b.sat[I] != 0;
-  b.skip[c] = all ? 1 : 0;
+  bool all = false;
+  if (c < b.nchunks) {
+    const int Ia = b.part[c];
+    const int Ib = c + 1 < b.nchunks ? b.part[c + 1] : b.nbr - 1;
+    all = Ib - Ia <= 32;
+    for (int I = Ia; all && I <= Ib; ++I) all = b.sat[I] != 0;
+    b.skip[c] = all ? 1 : 0;
+  }
+  if (__ballot(all) != 0 && (threadIdx.x & 63) == 0) b.ctl[5] = 1;
 }
 
 // Pushes frontier row r of A^T's block row I over its tiles [k0, k1) (lanes
