@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include "bgb_device.h"
@@ -93,16 +94,30 @@ __device__ __forceinline__ uint2 x_gather(const uint2 *xb, int J, int h, int lan
 
 // Processes tiles [k0, k1) of one graph; returns the row left open (or nbr).
 //
-// Groups of kGroup<T> tiles: the X gathers of group g are issued, then the row
-// words of group g+1 (block columns through SMEM), then the MFMAs of group g
-// wait for the gathers alone. Every load is issued unconditionally (indices
-// past the chunk are clamped to its last tile and their words zeroed), so the
-// compiler can count the loads in flight instead of draining them.
+// Groups of kGroup<T> tiles: the X gathers of group g are issued, then the
+// tile words of group g+1 (block columns through SMEM), then the MFMAs of
+// group g wait for the gathers alone. Every load is issued unconditionally
+// (indices past the chunk are clamped to its last tile and their words
+// zeroed), so the compiler can count the loads in flight instead of draining
+// them.
+//
+// A group's tile words (U*T row words, 256 B at T = 16, 1 KB at T = 32) come
+// in with one vector load per lane and go through a per-wave LDS buffer to
+// the lanes that use them (lane m takes row m of each tile): one VMEM
+// instruction per group instead of one per tile and row half. The vector
+// memory pipeline (address unit) is what bounds this kernel: every X gather
+// is one VMEM instruction per tile already.
 template <int T>
 __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane, float *carry) {
   constexpr int MH = T / 16, U = kGroup<T>;
   using W = typename Tile<T>::W;
-  const W *__restrict__ A = static_cast<const W *>(G.val);
+  constexpr int kTB = U * T * (int)sizeof(W);  // bytes of one group's tiles
+  constexpr int kLB = kTB / 64;                // of which each lane loads kLB
+  static_assert(kLB == 4 || kLB == 16, "group tile bytes per lane");
+  using GV = typename std::conditional<kLB == 4, uint32_t, uint4>::type;
+  __shared__ __align__(16) uint8_t s_tw[kWaves][kTB];
+  uint8_t *tw = s_tw[threadIdx.x >> 6];
+  const uint8_t *Ab = static_cast<const uint8_t *>(G.val);
   const int m = lane & 15, kq = lane >> 4;
   int I = __builtin_amdgcn_readfirstlane(wave_find_row(G.rowptr, 0, G.nbr - 1, k0, lane));
   float4_t D[MH], Z[MH];
@@ -114,19 +129,38 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
   int e = sload(G.rowptr, I + 1);
   int e2 = I + 1 < G.nbr ? sload(G.rowptr, I + 2) : INT_MAX;
   const long long last = k1 - 1;
-  uint32_t w[U][MH], wn[U][MH];
+  uint32_t w[U][MH];
+  GV gv, gvn;
   int J[U], Jn[U];
-  auto load_tiles = [&](long long p0, uint32_t (&ww)[U][MH], int (&jj)[U]) {
+  // Tiles p0 .. p0+U-1 (p0 a multiple of U past k0, so the group is kTB-byte
+  // aligned): the buffer resource ends at the tile array's end, so bytes past
+  // it -- a last partial group, or the prefetch after the chunk's last group
+  // -- read as zero without a memory access.
+  auto load_group = [&](long long p0, GV &g, int (&jj)[U]) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long long p = p0 + u < last ? p0 + u : last;
-      jj[u] = sload(G.colind, p);
-#pragma unroll
-      for (int h = 0; h < MH; ++h) ww[u][h] = A[p * T + h * 16 + m];
-    }
+    for (int u = 0; u < U; ++u) jj[u] = sload(G.colind, p0 + u < last ? p0 + u : last);
+    const long long rem = ((long long)G.nblocks - p0) * T * (long long)sizeof(W);
+    const int nrec = rem <= 0 ? 0 : (rem < kTB ? (int)rem : kTB);
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t *>(Ab + p0 * T * (long long)sizeof(W)), (short)0, nrec, 0x00020000);
+    if constexpr (kLB == 4) g = __builtin_amdgcn_raw_buffer_load_b32(r, lane * kLB, 0, 0);
+    else g = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, lane * kLB, 0, 0));
   };
-  load_tiles(k0, w, J);
+  // the group's row words from LDS: lane m, row h*16 + m of tile u
+  auto stage_group = [&](const GV &g) {
+    *reinterpret_cast<GV *>(tw + lane * kLB) = g;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const W *tws = reinterpret_cast<const W *>(tw);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int h = 0; h < MH; ++h) w[u][h] = tws[u * T + h * 16 + m];
+  };
+  load_group(k0, gv, J);
   for (long long p0 = k0; p0 < k1; p0 += U) {
+    stage_group(gv);
     // X rows of a 4-column group only matter where the tile has a set bit in
     // those columns (the A operand is 0 elsewhere): the tile's column mask is
     // the OR of its row words, which sit in one 16-lane DPP row (lane m holds
@@ -148,7 +182,7 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
         xv[u][h] = x_gather<MH>(G.xb, J[u], h, lane, grp != 0);
       }
     }
-    load_tiles(p0 + U, wn, Jn);
+    load_group(p0 + U, gvn, Jn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -190,12 +224,11 @@ __device__ int k16_chunk(const K16Graph &G, long long k0, long long k1, int lane
         }
       }
     }
+    gv = gvn;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      J[u] = Jn[u];
-#pragma unroll
-      for (int h = 0; h < MH; ++h) w[u][h] = wn[u][h];
-    }
+    for (int u = 0; u < U; ++u) J[u] = Jn[u];
+    // the next stage_group overwrites the buffer: every lane's reads of this
+    // group are done (LDS operations of a wave complete in order)
   }
   if (I < G.nbr) {
     const int rq = lane >> 4;
