@@ -60,6 +60,8 @@ struct BmvArgs {
   void *carry_val;
   int *part;  // part[c]: block row of chunk c's first tile (cached per stream and rowptr)
   int fixup_blocks;  // workgroups of the fix-up pass proper; any after them zero the empty block rows
+  int xl_words;      // LDS kernels: packed-x words [0, xl_words) staged, the rest read from B
+  int x_words;       // packed-x words in B (the range of its buffer resource)
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -68,7 +70,8 @@ constexpr int kWavesPerBlock = 4;
 // Longer runs put more bytes in flight per wave, which matters for the small
 // tiles; set with bgb_set_run_length.
 int g_run_length[5] = {8, 8, 8, 8, 8};
-// Packed-x staging per tile dim: 0 automatic, 1 never, 2 whenever it fits.
+// Packed-x staging per tile dim: 0 automatic, 1 never, 2 always (all of x
+// when it fits in LDS, else its first words, the rest read from memory).
 int g_x_staging[5] = {0, 0, 0, 0, 0};
 // Short-row path (T <= 8): 0 automatic (every block row at most kShortRowMax
 // tiles), 1 never, 2 always.
@@ -161,9 +164,10 @@ __device__ __forceinline__ void load_chunk(const BmvArgs &a, int chunk, int lane
 }
 
 // One chunk of CH tiles for one wave (all 64 lanes) from its loaded inputs:
-// `win` is the wave's LDS row window, `xl` the packed x staged in LDS (XL) or
-// unused (x gathered from memory).
-template <int T, int OP, int S, bool XL>
+// `win` is the wave's LDS row window, `xl` the packed x staged in LDS
+// (XL = 1: all of it; XL = 2: words [0, a.xl_words)) or unused (XL = 0: x
+// gathered from memory).
+template <int T, int OP, int S, int XL>
 __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane, int *win, const uint8_t *xl,
                                           const ChunkIn<T, S> &in) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
@@ -183,12 +187,32 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
   // issue back to back instead of one predicated region and wait per tile.
   Word<T> X[S];
   if constexpr (!is_full(OP)) {
+    // XL = 2: every lane reads LDS (its index clamped into the staged words)
+    // and memory through a buffer resource over x, with an out-of-range
+    // offset where the word is staged -- no memory access for those lanes,
+    // and both loads unconditional
+    __amdgpu_buffer_rsrc_t xr;
+    if constexpr (XL == 2)
+      xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.B), (short)0, a.x_words * (int)sizeof(W),
+                                             0x00020000);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const bool v = active && s < n;
       const int j = v ? J[s] : 0;
-      if constexpr (XL) X[s] = lds_xword<T>(xl, j);
-      else X[s] = (Word<T>) static_cast<const W *>(a.B)[j];
+      if constexpr (XL == 1) {
+        X[s] = lds_xword<T>(xl, j);
+      } else if constexpr (XL == 2) {
+        const bool staged = j < a.xl_words;
+        const Word<T> xs = lds_xword<T>(xl, staged ? j : 0);
+        const int off = staged ? a.x_words * (int)sizeof(W) : j * (int)sizeof(W);
+        Word<T> xm;
+        if constexpr (sizeof(W) == 1) xm = __builtin_amdgcn_raw_buffer_load_b8(xr, off, 0, 0);
+        else if constexpr (sizeof(W) == 2) xm = __builtin_amdgcn_raw_buffer_load_b16(xr, off, 0, 0);
+        else xm = __builtin_amdgcn_raw_buffer_load_b32(xr, off, 0, 0);
+        X[s] = staged ? xs : xm;
+      } else {
+        X[s] = (Word<T>) static_cast<const W *>(a.B)[j];
+      }
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) X[s] = (active && s < n) ? X[s] : (Word<T>)0;
@@ -523,7 +547,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void bmv_kernel(BmvArgs a) {
   ChunkIn<T, S> in;
   load_chunk<T, S>(a, chunk, lane, rows, in);
   (void)CH;
-  bmv_chunk<T, OP, S, false>(a, chunk, lane, s_win[threadIdx.x >> 6], nullptr, in);
+  bmv_chunk<T, OP, S, 0>(a, chunk, lane, s_win[threadIdx.x >> 6], nullptr, in);
 }
 
 // Raw partition entries of chunk c (part[c], part[c+1]) as vector loads, so
@@ -545,7 +569,9 @@ __device__ __forceinline__ int2 load_part_vec(const BmvArgs &a, int c) {
 // length 8; the larger fragments need the 256 VGPRs of 8-wave workgroups.
 template <int T, int S>
 constexpr int lds_waves() { return (T >= 16 || S > 8) ? 8 : 16; }
-template <int T, int OP, int S>
+// XL = 1: all of x staged (x_bytes of it); XL = 2: its first a.xl_words words
+// (x_bytes bytes), the other words read from memory.
+template <int T, int OP, int S, int XL>
 __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
   constexpr int kLdsWaves = lds_waves<T, S>();
   extern __shared__ __align__(16) uint8_t s_dyn[];
@@ -606,7 +632,7 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
     const int cl = more ? cn : chunk;
     load_chunk<T, S, true>(a, cl, lane, rows_of(cl, pnext), nxt);
     pnext = load_part_vec(a, cn + nw < nfull ? cn + nw : chunk);
-    bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, cur);
+    bmv_chunk<T, OP, S, XL>(a, chunk, lane, win, xl, cur);
     c = cn;
     return more;
   };
@@ -621,7 +647,7 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
     const int chunk = __builtin_amdgcn_readfirstlane(c);
     ChunkIn<T, S> in;
     load_chunk<T, S>(a, chunk, lane, rows_of(chunk, load_part_vec(a, chunk)), in);
-    bmv_chunk<T, OP, S, true>(a, chunk, lane, win, xl, in);
+    bmv_chunk<T, OP, S, XL>(a, chunk, lane, win, xl, in);
   }
 }
 
@@ -940,9 +966,10 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   }
   bool lds_path = false;
   if constexpr (!is_full(OP)) {
-    // Whole packed x in LDS when it fits and the tile stream is large next to
-    // staging it once per workgroup. The staged words cover every block
-    // column the matrix references (a wide matrix reads past ceil(NBR/T)*T).
+    // Packed x in LDS when the tile stream is large next to staging it once
+    // per workgroup: all of it when it fits, else its first words. The words
+    // cover every block column the matrix references (a wide matrix reads
+    // past ceil(NBR/T)*T).
     DeviceFacts f;
     if ((st = device_facts(&f))) return st;
     int maxcol = -1;
@@ -950,26 +977,47 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
     if (!scr && mode != 1 && (mode == 2 || T <= 16))
       if ((st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
     const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
-    const long long x_bytes = words * (long long)sizeof(typename Tile<T>::W);
-    const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;
+    using W = typename Tile<T>::W;
+    const long long x_bytes = words * (long long)sizeof(W);
+    const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;  // LDS bytes of all of x
     constexpr int kLdsWaves = lds_waves<T, S>();
-    const long long lds = (long long)sizeof(int) * kLdsWaves * kWinAlloc + xl;
-    const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(typename Tile<T>::W));
+    const long long win_bytes = (long long)sizeof(int) * kLdsWaves * kWinAlloc;
+    const bool fits = win_bytes + xl <= f.lds_per_block;
+    // x larger than LDS: its first words are staged (as many as the LDS left
+    // after the row windows holds, in whole 16-byte units), the others are
+    // read from memory. R-MAT graphs reference their low block columns most
+    // (the first quarter of the columns of R-MAT 22 holds about 55 % of the
+    // tiles).
+    const long long part_lds = (f.lds_per_block - win_bytes) & ~15ll;
+    const long long part_words = T == 4 ? 2 * part_lds : part_lds / (long long)sizeof(W);
+    const bool partial_ok = !fits && part_lds >= 16384 && x_bytes < (1ll << 31);
+    const long long stage_lds = fits ? xl : part_lds;
+    const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(W));
     const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
     // automatic: T <= 16 only (T = 32 streams 132 B per tile and is
     // bandwidth-bound; the persistent grid's lower occupancy costs more there
     // than the x gathers it saves, measured on MI355X)
-    const bool worth = mode == 2 || (mode == 0 && T <= 16 && tile_bytes >= 2 * xl * f.cus);
-    if (maxcol >= 0 && aligned && lds <= f.lds_per_block && worth) {
-      static bool attr_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S>),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)f.lds_per_block) == hipSuccess;
+    const bool worth = mode == 2 || (mode == 0 && T <= 16 && tile_bytes >= 2 * stage_lds * f.cus);
+    if (maxcol >= 0 && aligned && (fits || partial_ok) && worth) {
+      static bool attr_ok[2] = {
+          hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)f.lds_per_block) == hipSuccess,
+          hipFuncSetAttribute(reinterpret_cast<const void *>(&bmv_lds_kernel<T, OP, S, 2>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)f.lds_per_block) == hipSuccess};
       (void)hipGetLastError();
-      if (attr_ok) {
+      if (attr_ok[fits ? 0 : 1]) {
+        const long long lds = win_bytes + stage_lds;
         const int per_cu = std::max(1, std::min(2, f.lds_per_cu / (int)lds));
         const long long want = ceil_div(nchunks, kLdsWaves);
         const unsigned grid = (unsigned)std::min<long long>((long long)f.cus * per_cu, want);
-        bmv_lds_kernel<T, OP, S><<<grid, 64 * kLdsWaves, (size_t)lds, s>>>(a, (int)x_bytes);
+        if (fits) {
+          bmv_lds_kernel<T, OP, S, 1><<<grid, 64 * kLdsWaves, (size_t)lds, s>>>(a, (int)x_bytes);
+        } else {
+          a.xl_words = (int)part_words;
+          a.x_words = (int)words;
+          bmv_lds_kernel<T, OP, S, 2><<<grid, 64 * kLdsWaves, (size_t)lds, s>>>(
+              a, (int)(part_words * (long long)sizeof(W)));
+        }
         BGB_HIP_TRY(hipGetLastError());
         lds_path = true;
       }

@@ -373,3 +373,33 @@ def test_x_staging_modes(orc, dev, T):
                 check_all_ops(orc, csr, T, rng, dev, f"x staging mode {mode} {label}")
     finally:
         bgb.call("bgb_set_x_staging", T, 0)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("T", (4, 8, 16))
+def test_partial_x_staging(orc, dev, T):
+    """A packed x larger than LDS (1.5 M vertices): the LDS kernels stage its
+    first words and read the rest from memory. Columns on both sides of the
+    staged prefix -- an R-MAT-like skew toward low ids plus uniform ones, a
+    hub row and a hub column, and a wide matrix whose extra block columns lie
+    past the square part -- against the oracle, staging forced and automatic."""
+    import bgb
+    from bgb import graph
+    rng = np.random.default_rng(90 + T)
+    n = 1500000
+    m = 3000000
+    src = rng.integers(0, n, m)
+    dst = np.where(rng.random(m) < 0.6, rng.integers(0, n // 5, m), rng.integers(0, n, m))
+    hub = np.full(20000, 7)
+    far = rng.integers(0, n, 20000)
+    src = np.concatenate([src, hub, far]).astype(np.int32)
+    dst = np.concatenate([dst, far, np.full(20000, n - 3)]).astype(np.int32)
+    sq = graph.from_coo(n, n, src, dst, False, True)
+    wide = graph.from_coo(400000, n, src % 400000, dst, False, True)
+    try:
+        for mode in (2, 0):
+            bgb.call("bgb_set_x_staging", T, mode)
+            for label, csr in (("square", sq), ("wide 400000 x 1500000", wide)):
+                check_all_ops(orc, csr, T, rng, dev, f"partial x staging, mode {mode}, {label}")
+    finally:
+        bgb.call("bgb_set_x_staging", T, 0)
