@@ -990,7 +990,10 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
     // tiles).
     const long long part_lds = (f.lds_per_block - win_bytes) & ~15ll;
     const long long part_words = T == 4 ? 2 * part_lds : part_lds / (long long)sizeof(W);
-    const bool partial_ok = !fits && part_lds >= 16384 && x_bytes < (1ll << 31);
+    // automatic: only while the staged words are at least an eighth of x
+    // (uniformly spread columns gain little from a small prefix)
+    const bool partial_ok =
+        !fits && part_lds >= 16384 && x_bytes < (1ll << 31) && (mode == 2 || 8 * part_words >= words);
     const long long stage_lds = fits ? xl : part_lds;
     const long long tile_bytes = (long long)a.nblocks * (4 + T * (long long)sizeof(W));
     const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;

@@ -130,11 +130,22 @@ const PathFit kBitMem = {15.0, {4.3, 4.6, 9.3, 24.0}};
 const PathFit kFpShort = {4.0, {3.5, 4.4, 0, 0}};
 const PathFit kFpChunk = {12.0, {6.0, 7.7, 27.0, 52.0}};
 
-bool lds_path(int T, const bgb_tile_profile &p, int cus = 256) {
+// Bits from memory past the staged first words of a packed x larger than
+// LDS (R-MAT 22: about 55 % of the tiles' block columns staged).
+const PathFit kBitPart = {15.0, {3.25, 3.7, 8.3, 0}};
+
+// Packed-x placement of the bit op: 0 gathered from memory, 1 all of it in
+// LDS, 2 its first words in LDS (the launch rule of the BMV kernels: T <= 16
+// and a tile stream at least twice the LDS copies of one staging per CU).
+int lds_path(int T, const bgb_tile_profile &p, int cus = 256) {
   const int t = T == 4 ? 0 : T == 8 ? 1 : T == 16 ? 2 : 3;
-  if (T > 16 || lds_x_bytes(T, p.nrows) > kLdsForX) return false;
+  if (T > 16) return 0;
+  const bool fits = lds_x_bytes(T, p.nrows) <= kLdsForX;
+  const double staged = fits ? (double)lds_x_bytes(T, p.nrows) : (double)kLdsForX;
   const double wb = T == 16 ? 2 : 1;
-  return (double)p.tiles[t] * (4.0 + T * wb) >= 2.0 * (double)lds_x_bytes(T, p.nrows) * cus;
+  if ((double)p.tiles[t] * (4.0 + T * wb) < 2.0 * staged * cus) return 0;
+  if (!fits && 8.0 * (double)kLdsForX < (double)lds_x_bytes(T, p.nrows)) return 0;  // prefix under 1/8 of x
+  return fits ? 1 : 2;
 }
 
 // Predicted microseconds of one call of op at tile dim index t, run length S.
@@ -151,7 +162,7 @@ double predict_us(int op, int t, int S, const bgb_tile_profile &p, int *x_stagin
     const PathFit *f;
     if (short_rows) f = op == 0 ? &kBitShort : &kFpShort;
     else if (op == 0 && lds_path(T, p)) {
-      f = &kBitLds;
+      f = lds_path(T, p) == 1 ? &kBitLds : &kBitPart;
       if (x_staging) *x_staging = 2;
     } else f = op == 0 ? &kBitMem : &kFpChunk;
     double a = f->per_mtile[t];
