@@ -153,18 +153,13 @@ __device__ __forceinline__ void mask_cols(uint32_t (&F)[Tile<T>::NW], uint32_t x
 // round trip the next tile waits for.
 constexpr int kVisStage = 512;
 
+// One chunk of the pull pass for one wave: `win` and `sv` are the wave's LDS
+// row window and visited-word stage.
 template <int T, int S>
-__global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
+__device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane, int depth, int *win,
+                                          uint64_t *sv) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
-  __shared__ int s_win[kBfsWaves][kWinAlloc];
-  __shared__ uint64_t s_vis[kBfsWaves][kVisStage];
-  if (b.ctl[2] || b.ctl[3]) return;  // BFS finished, or a push level: no-op
-  const int depth = b.ctl[0];
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int chunk = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kBfsWaves + wid);
-  if (chunk >= b.nchunks) return;
   // Every block row of the chunk saturated (bfs_chunk_skip_kernel): nothing
   // can be new here and the rows' next-frontier words are already cleared.
   // Tested before the tiles are loaded, so a skipped chunk reads none of
@@ -207,7 +202,6 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   const long long vbase = (long long)Ia * T;
   const int vcount = (Ib - Ia + 1) * T;
   const bool vst = vcount <= kVisStage && Ib + 2 - Ia <= kWin;  // wave-uniform
-  uint64_t *sv = s_vis[wid];
   if (vst) {
     uint64_t vt[kVisStage / 64];
 #pragma unroll
@@ -218,7 +212,7 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
 #pragma unroll
     for (int j = 0; j < kVisStage / 64; ++j) sv[lane + 64 * j] = vt[j];
   }
-  rw.put(rowptr, s_win[wid], Ia, Ib, lane, wv);  // (its wave barrier also covers the stage)
+  rw.put(rowptr, win, Ia, Ib, lane, wv);  // (its wave barrier also covers the stage)
   const bool sym = b.rowptrT == rowptr;  // A^T's block rows are A's: lengths from the window
   // The walk, compiled for: row ends and visited words in LDS (the common
   // case), row ends in LDS and visited words in memory, both in memory.
@@ -350,6 +344,29 @@ __global__ __launch_bounds__(64 * kBfsWaves) void bfs_level_kernel(BfsState b) {
   if (!__builtin_amdgcn_readfirstlane((int)rw.staged)) walk(std::false_type{}, std::false_type{});
   else if (__builtin_amdgcn_readfirstlane((int)vst)) walk(std::true_type{}, std::true_type{});
   else walk(std::true_type{}, std::false_type{});
+}
+
+// The pull pass: a persistent grid whose waves walk chunks w, w + nw, ...
+// (nw waves in the grid). A one-chunk-per-wave launch left most of the
+// machine idle: a workgroup's slot stays taken until its slowest wave ends,
+// and chunk times range from under a microsecond (skipped) to tens of
+// microseconds (hundreds of row changes). Push levels and the levels after
+// the last one return at once.
+// T = 4 is held to 96 VGPRs (5 waves per SIMD, a few bytes of spill): 5.69 ->
+// 5.53 ms on R-MAT 22 against 104 VGPRs at 4 waves.
+template <int T, int S>
+__global__ __launch_bounds__(64 * kBfsWaves) __attribute__((amdgpu_waves_per_eu(T == 4 ? 5 : 1)))
+void bfs_level_kernel(BfsState b) {
+  __shared__ int s_win[kBfsWaves][kWinAlloc];
+  __shared__ uint64_t s_vis[kBfsWaves][kVisStage];
+  if (b.ctl[2] || b.ctl[3]) return;  // BFS finished, or a push level: no-op
+  const int depth = b.ctl[0];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = __builtin_amdgcn_readfirstlane((int)gridDim.x * kBfsWaves);
+  for (int chunk = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kBfsWaves + wid); chunk < b.nchunks;
+       chunk += nw)
+    bfs_chunk<T, S>(b, chunk, lane, depth, s_win[wid], s_vis[wid]);
 }
 
 // Folds the carries of rows that cross chunks into this level's results. One
@@ -784,7 +801,15 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     b.sat = sat.as<uint8_t>();
     b.skip = skp.as<uint8_t>();
   }
-  const unsigned blocks = (unsigned)ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves);
+  // pull pass: a persistent grid of the workgroups the GPU holds at once
+  // (every wave gets the same share of chunks, so none may wait for a slot)
+  DeviceFacts f;
+  if (int e = device_facts(&f)) return e;
+  int per_cu = 0;
+  BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(&bfs_level_kernel<T, S>), 64 * kBfsWaves, 0));
+  const unsigned blocks = (unsigned)std::max<long long>(
+      1, std::min<long long>(ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves), (long long)f.cus * std::max(per_cu, 1)));
   const unsigned fblocks = nblk(ceil_div(nchunks, 64) * 64);  // one lane per chunk, whole waves
   const unsigned pblocks = (unsigned)std::min<long long>(ceil_div(nbr, 4), 8192);
   const unsigned long long push_limit = push ? (unsigned long long)nblocks / kPushDiv : 0ull;
