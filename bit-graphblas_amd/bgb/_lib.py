@@ -66,6 +66,8 @@ SIGNATURES = {
     "bgb_bfs_multi_dir": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_int,
                                    _vp, _c_int, _vp, _vp]),
     "bgb_set_bfs_level_timing": (_c_int, [_c_int]),
+    "bgb_set_bfs_gather_mode": (_c_int, [_c_int]),
+    "bgb_get_bfs_gather_mode": (_c_int, []),
     "bgb_bfs_level_times": (_c_int, [_c_int, _vp, _vp, _vp]),
     "bgb_bfs_plain": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "bgb_tile_stats": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),

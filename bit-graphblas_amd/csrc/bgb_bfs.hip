@@ -99,12 +99,16 @@ __device__ __forceinline__ void publish_live(const BfsState &b, uint64_t ln, int
 // read-modify-write would make the wave wait for the read.
 __device__ __forceinline__ bool record_new(const BfsState &b, long long v, uint64_t vold, uint64_t nb,
                                            int depth) {
-  b.Fn[v] = nb;
+  // Fn starts the level cleared (bfs_clear_kernel): only new bits are stored
   if (nb) {
+    b.Fn[v] = nb;
     b.V[v] = vold | nb;
-#pragma unroll
-    for (int k = 0; k < kPlanes; ++k)
-      if ((depth >> k) & 1) atomicOr(reinterpret_cast<unsigned long long *>(b.P + k * b.npad + v), nb);
+    // one atomic per set bit of the depth (a loop, not 16 unrolled sites:
+    // this runs at every row end of the unrolled walk, and the unrolled form
+    // made the level kernel larger than the instruction cache)
+#pragma unroll 1
+    for (int m = depth; m; m &= m - 1)
+      atomicOr(reinterpret_cast<unsigned long long *>(b.P + (__ffs(m) - 1) * b.npad + v), nb);
     return true;
   }
   return false;
@@ -133,7 +137,7 @@ __device__ __forceinline__ void bfs_flush(const BfsState &b, int I, int q, const
   if (b.rowptrT && bits) cost += (unsigned long long)__popc(bits) * row_len(I);
 #pragma unroll
   for (int m = 1; m < L; m <<= 1) bits |= (uint32_t)__shfl_xor((int)bits, m);
-  if (q == 0) static_cast<W *>(b.fnzn)[I] = (W)bits;
+  if (q == 0 && bits) static_cast<W *>(b.fnzn)[I] = (W)bits;
 }
 
 // Row words of a tile restricted to the columns whose frontier word is
@@ -155,20 +159,44 @@ constexpr int kVisStage = 512;
 
 // One chunk of the pull pass for one wave: `win` and `sv` are the wave's LDS
 // row window and visited-word stage.
+// kG: 0 gathers the frontier word of each set bit of each row in turn, the
+// loop stopping as soon as the row is saturated; 1 (T <= 8, one lane per
+// tile) gathers once per set column of the tile and ORs the word into every
+// row holding that column -- one gather instruction per column instead of
+// one per row, with more lanes active in each (the gathers are bound by the
+// address units, not by bytes).
 template <int T, int S>
+struct BfsIn {
+  uint32_t F[S][Tile<T>::NW];
+  int J[S];
+  int wv[kWin / 64];
+  int Ia, Ib;  // block rows of the chunk's first tile and of the next chunk's
+};
+
+// Issues the loads of chunk `chunk` (rows Ia .. Ib, wave-uniform): its tile
+// runs, block columns and row-window entries. kFull: every lane group has a
+// whole run of S tiles (the loads are then unconditional, which keeps the
+// compiler's count of loads in flight exact across the pipelined walk).
+template <int T, int S, bool kFull>
+__device__ __forceinline__ void bfs_load(const BfsState &b, int chunk, int lane, int Ia, int Ib,
+                                         BfsIn<T, S> &in) {
+  constexpr int L = Tile<T>::L, G = 64 / L, CH = G * S;
+  const int g = lane / L, q = lane % L;
+  const long long k0 = (long long)chunk * CH;
+  const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
+  const long long pb = lmin2(pa + S, b.nblocks);
+  if (kFull) load_run<T, S, true>(b.A, b.colind, pa, S, q, in.F, in.J);
+  else if (pa < pb) load_run<T, S>(b.A, b.colind, pa, (int)(pb - pa), q, in.F, in.J);
+  RowWindow::load(b.rowptr, Ia, Ib, lane, in.wv);
+  in.Ia = Ia;
+  in.Ib = Ib;
+}
+
+template <int T, int S, int kG>
 __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane, int depth, int *win,
-                                          uint64_t *sv) {
+                                          uint64_t *sv, const BfsIn<T, S> &in) {
   constexpr int L = Tile<T>::L, R = Tile<T>::R, NW = Tile<T>::NW, G = 64 / L, CH = G * S;
   using W = typename Tile<T>::W;
-  // Every block row of the chunk saturated (bfs_chunk_skip_kernel): nothing
-  // can be new here and the rows' next-frontier words are already cleared.
-  // Tested before the tiles are loaded, so a skipped chunk reads none of
-  // them -- at levels where some chunk is skipped (ctl[5]; the others load
-  // their tiles at once).
-  if (b.ctl[5] && b.skip[chunk]) {
-    if (lane == 0) b.carry_key[chunk] = -1;
-    return;
-  }
   const int g = lane / L, q = lane % L;
   const long long k0 = (long long)chunk * CH;
   const long long pa = lmin2(k0 + (long long)g * S, b.nblocks);
@@ -177,13 +205,16 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
   const int *__restrict__ rowptr = b.rowptr;
   uint32_t F[S][NW];
   int J[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    J[s] = in.J[s];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) F[s][w] = in.F[s][w];
+  }
   const int n = (int)(pb - pa);
-  if (active) load_run<T, S>(b.A, b.colind, pa, n, q, F, J);
-  const int Ia = b.part[chunk];
-  const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
+  const int Ia = in.Ia, Ib = in.Ib;
   const uint64_t dead = ~*b.live;  // sources whose BFS has no frontier left
   RowWindow rw;
-  int wv[kWin / 64];
   // Only columns with a non-empty frontier word are gathered: the tile words
   // are masked by the packed frontier-non-empty words of their block columns
   // (gathered unconditionally -- block column 0 for missing tiles, whose
@@ -195,7 +226,6 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
 #pragma unroll
     for (int s = 0; s < S; ++s) mask_cols<T>(F[s], (active && s < n) ? X[s] : 0u);
   }
-  RowWindow::load(rowptr, Ia, Ib, lane, wv);
   // visited words of vertices [Ia*T, (Ib+1)*T), when they fit the wave's
   // stage (and the row window is staged): unconditional loads, indices
   // clamped into the range
@@ -203,15 +233,23 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
   const int vcount = (Ib - Ia + 1) * T;
   const bool vst = vcount <= kVisStage && Ib + 2 - Ia <= kWin;  // wave-uniform
   if (vst) {
-    uint64_t vt[kVisStage / 64];
+    // as many 64-vertex loads as the chunk's vertices need (most chunks lie
+    // in one or a few block rows: one load instead of kVisStage / 64)
+    auto stage = [&](auto nl) {
+      constexpr int NL = decltype(nl)::value;
+      uint64_t vt[NL];
 #pragma unroll
-    for (int j = 0; j < kVisStage / 64; ++j) {
-      const int i = lane + 64 * j;
-      vt[j] = b.V[vbase + (i < vcount ? i : vcount - 1)];
-    }
+      for (int j = 0; j < NL; ++j) {
+        const int i = lane + 64 * j;
+        vt[j] = b.V[vbase + (i < vcount ? i : vcount - 1)];
+      }
 #pragma unroll
-    for (int j = 0; j < kVisStage / 64; ++j) sv[lane + 64 * j] = vt[j];
+      for (int j = 0; j < NL; ++j) sv[lane + 64 * j] = vt[j];
+    };
+    if (vcount <= 64) stage(std::integral_constant<int, 1>{});
+    else stage(std::integral_constant<int, kVisStage / 64>{});
   }
+  const int(&wv)[kWin / 64] = in.wv;
   rw.put(rowptr, win, Ia, Ib, lane, wv);  // (its wave barrier also covers the stage)
   const bool sym = b.rowptrT == rowptr;  // A^T's block rows are A's: lengths from the window
   // The walk, compiled for: row ends and visited words in LDS (the common
@@ -245,6 +283,22 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
 #pragma unroll
   for (int i = 0; i < R; ++i) acc[i] = hv[i] = 0;
   if (active) load_vis(I, vr);
+  // kG = 1 (one lane per tile): byte r of om[j] is 0xFF while row 4j+r of the
+  // open block row is not saturated -- recomputed only when the partial or
+  // the visited words change (a gather, a new row), so a tile's column set is
+  // a few bitwise ops
+  constexpr int kOM = (kG == 1 && L == 1) ? NW : 1;
+  uint32_t om[kOM];
+  auto update_open = [&]() {
+    if constexpr (kG == 1 && L == 1) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) om[j] = 0;
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        if ((acc[i] | vr[i] | dead) != ~0ull) om[i >> 2] |= 0xFFu << (8 * (i & 3));
+    }
+  };
+  update_open();
   int held = -1;
   // end of row I and, read one row ahead, of row I+1 (indices clamped into the
   // window / rowptr; clamped ends belong to rows past the chunk)
@@ -252,21 +306,104 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
   auto row_end = [&](int r) { return r < b.nbr ? rw.at_t<ST>(r + 1 < hi ? r + 1 : hi) : INT_MAX; };
   int e = active ? row_end(I) : INT_MAX;
   int e2 = active ? row_end(I + 1) : INT_MAX;
+  // kG = 1: the frontier word of each tile's highest set column is gathered
+  // for a group of kPre tiles at once, before the group is walked (through a
+  // buffer resource over F, with an offset past its end -- no memory access
+  // -- for tiles without a frontier column): one round trip per group
+  // instead of one per tile position. Further columns of a tile (few) are
+  // gathered in the walk.
+  constexpr int kPre = (kG == 1 && L == 1) ? 4 : 1;
+  __amdgpu_buffer_rsrc_t frs;
+  int f_end = 0;
+  if constexpr (kG == 1 && L == 1) {
+    f_end = (int)(b.npad * 8);
+    frs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t *>(b.F), (short)0, f_end, 0x00020000);
+  }
+  // column set of tile s: its frontier columns in rows not saturated yet
+  // (om describes the open row; a tile past its end starts a new row)
+  auto tile_cols = [&](const uint32_t (&Fw)[NW], bool in_open_row) -> uint32_t {
+    uint32_t cw = 0;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
+    for (int j = 0; j < (kG == 1 && L == 1 ? NW : 1); ++j) cw |= Fw[j] & (in_open_row ? om[j] : ~0u);
+    cw |= cw >> 16;
+    cw |= cw >> 8;
+    return cw & ((1u << (T < 32 ? T : 31)) - 1u);
+  };
+  // The tiles in a loop over groups of kPre, not unrolled: the fully
+  // unrolled walk, with a row flush at every tile, was several times the
+  // instruction cache.
+  static_assert(S % kPre == 0, "groups must tile the run");
+#pragma unroll 1
+  for (int s0 = 0; s0 < S; s0 += kPre) {
+    // this group's tile words and block columns, picked with static indices
+    // (a dynamic index into the run's arrays sends them to scratch)
+    uint32_t Fq[kPre][NW];
+    int Jq[kPre];
+#pragma unroll
+    for (int g2 = 0; g2 < kPre; ++g2) {
+      if constexpr (kPre == 1) {
+        Jq[g2] = J[s0];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) Fq[g2][w] = F[s0][w];
+      } else {
+        Jq[g2] = J[g2];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) Fq[g2][w] = F[g2][w];
+#pragma unroll
+        for (int k = 1; k < S / kPre; ++k)
+          if (s0 == k * kPre) {
+            Jq[g2] = J[k * kPre + g2];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) Fq[g2][w] = F[k * kPre + g2][w];
+          }
+      }
+    }
+    uint64_t fg[kPre];
+    int fh[kPre];
+    if constexpr (kG == 1 && L == 1) {
+#pragma unroll
+      for (int g2 = 0; g2 < kPre; ++g2) {
+        const int t = s0 + g2;
+        const uint32_t cols = t < n ? tile_cols(Fq[g2], pa + t < (long long)e) : 0u;
+        const int h = cols ? 31 - __clz(cols) : -1;
+        const int off = cols ? (Jq[g2] * T + (T - 1 - h)) * 8 : f_end;
+        fg[g2] = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(frs, off, 0, 0));
+        fh[g2] = h;
+      }
+    }
+#pragma unroll
+  for (int g2 = 0; g2 < kPre; ++g2) {
+    const int s = s0 + g2;
     if (s < n) {
-      const uint64_t *__restrict__ fr = b.F + (long long)J[s] * T;
+      const uint64_t *__restrict__ fr = b.F + (long long)Jq[g2] * T;
+      if constexpr (kG == 1 && L == 1) {
+        // One gather per set column of the tile (the union of its rows that
+        // are not saturated yet), OR-ed into every row holding the column;
+        // the highest column's word was gathered with the group.
+        uint32_t cols = tile_cols(Fq[g2], true);
+        if (cols) {
+          do {
+            const int h = 31 - __clz(cols);
+            cols &= ~(1u << h);
+            const uint64_t f = h == fh[g2] ? fg[g2] : fr[T - 1 - h];
 #pragma unroll
-      for (int i = 0; i < R; ++i) {
-        uint32_t w = row_word<T>(F[s], i);
-        // Stop once the partial covers every live source that has not
-        // reached the vertex yet: frontier words only hold live sources, so
-        // more ORs cannot change acc & ~visited (this includes vertices every
-        // live source has reached).
-        while (w && (acc[i] | vr[i] | dead) != ~0ull) {
-          const int h = 31 - __clz(w);
-          acc[i] |= fr[T - 1 - h];
-          w &= ~(1u << h);
+            for (int i = 0; i < R; ++i) acc[i] |= ((row_word<T>(Fq[g2], i) >> h) & 1u) ? f : 0ull;
+          } while (cols);
+          update_open();
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          uint32_t w = row_word<T>(Fq[g2], i);
+          // Stop once the partial covers every live source that has not
+          // reached the vertex yet: frontier words only hold live sources, so
+          // more ORs cannot change acc & ~visited (this includes vertices every
+          // live source has reached).
+          while (w && (acc[i] | vr[i] | dead) != ~0ull) {
+            const int h = 31 - __clz(w);
+            acc[i] |= fr[T - 1 - h];
+            w &= ~(1u << h);
+          }
         }
       }
       const int pos = (int)(pa + s) + 1;
@@ -307,8 +444,10 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
           }
         }
         if (I < b.nbr) load_vis(I, vr);
+        update_open();
       }
     }
+  }
   }
   const int key = active ? I : INT_MAX;
   uint64_t sc[R];
@@ -350,23 +489,39 @@ __device__ __forceinline__ void bfs_chunk(const BfsState &b, int chunk, int lane
 // (nw waves in the grid). A one-chunk-per-wave launch left most of the
 // machine idle: a workgroup's slot stays taken until its slowest wave ends,
 // and chunk times range from under a microsecond (skipped) to tens of
-// microseconds (hundreds of row changes). Push levels and the levels after
-// the last one return at once.
-// T = 4 is held to 96 VGPRs (5 waves per SIMD, a few bytes of spill): 5.69 ->
-// 5.53 ms on R-MAT 22 against 104 VGPRs at 4 waves.
-template <int T, int S>
-__global__ __launch_bounds__(64 * kBfsWaves) __attribute__((amdgpu_waves_per_eu(T == 4 ? 5 : 1)))
+// microseconds (hundreds of row changes). Chunks whose block rows are all
+// saturated (bfs_chunk_skip_kernel, levels where ctl[5] is set) are neither
+// loaded nor walked. Push levels and the levels after the last one return at
+// once. (Loading the wave's next chunk while walking this one, as the BMV's
+// persistent kernels do, measured no faster here: the extra registers cost a
+// wave per SIMD.)
+// T = 4 is held to 96 VGPRs (5 waves per SIMD).
+template <int T, int S, int kG>
+__global__ __launch_bounds__(64 * kBfsWaves) __attribute__((amdgpu_waves_per_eu(T == 4 ? (kG == 1 ? 4 : 5) : 1)))
 void bfs_level_kernel(BfsState b) {
   __shared__ int s_win[kBfsWaves][kWinAlloc];
   __shared__ uint64_t s_vis[kBfsWaves][kVisStage];
   if (b.ctl[2] || b.ctl[3]) return;  // BFS finished, or a push level: no-op
   const int depth = b.ctl[0];
+  const bool skipping = b.ctl[5] != 0;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int nw = __builtin_amdgcn_readfirstlane((int)gridDim.x * kBfsWaves);
   for (int chunk = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kBfsWaves + wid); chunk < b.nchunks;
-       chunk += nw)
-    bfs_chunk<T, S>(b, chunk, lane, depth, s_win[wid], s_vis[wid]);
+       chunk += nw) {
+    // every block row of the chunk saturated: nothing can be new here and the
+    // rows' next-frontier words are already cleared; tested before the tiles
+    // are loaded, so a skipped chunk reads none of them
+    if (skipping && b.skip[chunk]) {
+      if (lane == 0) b.carry_key[chunk] = -1;
+      continue;
+    }
+    const int Ia = b.part[chunk];
+    const int Ib = chunk + 1 < b.nchunks ? b.part[chunk + 1] : b.nbr - 1;
+    BfsIn<T, S> in;
+    bfs_load<T, S, false>(b, chunk, lane, Ia, Ib, in);
+    bfs_chunk<T, S, kG>(b, chunk, lane, depth, s_win[wid], s_vis[wid], in);
+  }
 }
 
 // Folds the carries of rows that cross chunks into this level's results. One
@@ -443,11 +598,10 @@ __global__ void bfs_advance_kernel(int *ctl, unsigned long long *fcnt, unsigned 
 
 // Start of a level. Push levels write the next
 // frontier and its packed non-empty bits with atomics: both start cleared.
-// A pull level overwrites every entry of the non-empty rows it walks; here it
-// clears the next-frontier words of the block rows without tiles and of the
-// saturated ones -- every vertex already reached by every live source, so
-// nothing can be new there -- and marks the latter, so the level kernel may
-// skip chunks made of such rows.
+// A pull level stores only the new bits of the rows it walks: here every
+// next-frontier word and packed word is cleared, and block rows every live
+// source has reached (saturated: nothing can be new there) are marked, so the
+// level kernel may skip chunks made of such rows.
 template <int T>
 __global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
   using W = typename Tile<T>::W;
@@ -473,14 +627,14 @@ __global__ void bfs_clear_kernel(BfsState b, long long zbytes) {
     const unsigned long long m = __ballot(full);
     const unsigned long long grp = (T == 64 ? ~0ull : ((1ull << T) - 1)) << (lane & ~(T - 1));
     const bool row_sat = (m & grp) == grp;
-    // block rows without tiles: the level kernel steps over them, so their
-    // next-frontier words are cleared here (a source's seed bit included)
+    // every next-frontier word and packed word starts the level cleared (a
+    // source's seed bit included): the level kernel stores only new bits,
+    // and steps over block rows without tiles and skips saturated chunks
     const long long I = in ? v / T : 0;
-    const bool empty = in && b.rowptr[I] == b.rowptr[I + 1];
-    if (in && (row_sat || empty)) b.Fn[v] = 0;
+    if (in) b.Fn[v] = 0;
     if (in && (lane & (T - 1)) == 0) {
       b.sat[I] = row_sat ? 1 : 0;
-      if (row_sat || empty) static_cast<W *>(b.fnzn)[I] = (W)0;
+      static_cast<W *>(b.fnzn)[I] = (W)0;
     }
   }
 }
@@ -693,6 +847,10 @@ __global__ void plain_seed_kernel(int src, void *front, void *visited, int *leve
 
 unsigned nblk(long long n, int b = 256) { return (unsigned)ceil_div(n > 0 ? n : 1, b); }
 
+// Frontier gathers of the pull levels at T <= 8: 0 one per set bit of each
+// tile row, 1 one per set column of each tile (bgb_set_bfs_gather_mode).
+int g_bfs_gather = 1;
+
 // Per-level timing of the eager level loop (bgb_set_bfs_level_timing): the
 // GPU time of each level of the last timed call and whether it pulled.
 bool g_level_timing = false;
@@ -805,9 +963,14 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
   // (every wave gets the same share of chunks, so none may wait for a slot)
   DeviceFacts f;
   if (int e = device_facts(&f)) return e;
+  // frontier gathers per set column of a tile (T <= 8; F within a buffer
+  // resource's 32-bit byte range) or per set bit of a row
+  void (*level_kernel)(BfsState) = (T <= 8 && g_bfs_gather == 1 && 8 * npad < (1ll << 31))
+                                       ? &bfs_level_kernel<T, S, (T <= 8 ? 1 : 0)>
+                                       : &bfs_level_kernel<T, S, 0>;
   int per_cu = 0;
   BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(&bfs_level_kernel<T, S>), 64 * kBfsWaves, 0));
+      &per_cu, reinterpret_cast<const void *>(level_kernel), 64 * kBfsWaves, 0));
   const unsigned blocks = (unsigned)std::max<long long>(
       1, std::min<long long>(ceil_div(nchunks > 0 ? nchunks : 1, kBfsWaves), (long long)f.cus * std::max(per_cu, 1)));
   const unsigned fblocks = nblk(ceil_div(nchunks, 64) * 64);  // one lane per chunk, whole waves
@@ -818,7 +981,7 @@ int bfs_multi_impl(const void *A, const int *rowptr, const int *colind, int nbr,
     bfs_clear_kernel<T><<<nfill(npad), 256, 0, s>>>(st[p], zbytes);
     if (nchunks > 0) {
       bfs_chunk_skip_kernel<<<nblk(nchunks), 256, 0, s>>>(st[p]);
-      bfs_level_kernel<T, S><<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
+      level_kernel<<<blocks, 64 * kBfsWaves, 0, s>>>(st[p]);
       bfs_fixup_kernel<T><<<fblocks, 256, 0, s>>>(st[p]);
       if (push) {
         bfs_push_kernel<T><<<pblocks, 256, 0, s>>>(st[p]);
@@ -1013,6 +1176,15 @@ int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_
     }
   });
 }
+
+int bgb_set_bfs_gather_mode(int mode) {
+  clear_error();
+  BGB_REQUIRE(mode == 0 || mode == 1, BGB_ERR_ARG, "gather mode must be 0 or 1 (got %d)", mode);
+  g_bfs_gather = mode;
+  return BGB_OK;
+}
+
+int bgb_get_bfs_gather_mode(void) { return g_bfs_gather; }
 
 int bgb_set_bfs_level_timing(int on) {
   clear_error();

@@ -289,6 +289,14 @@ int bgb_bfs_multi_dir(int T, const void *d_A, const int *d_rowptr, const int *d_
                       const int *d_colindT, int n, const int *h_sources, int nsrc, int *d_levels,
                       int use_graph, int *h_nlevels, bgb_stream_t stream);
 
+/* Tuning knob (no reference counterpart): frontier gathers of the pull levels
+ * at T <= 8. 0 = one gather per set bit of each tile row in turn (the loop
+ * ending once the row is saturated); 1 = one gather per set column of a tile
+ * (over the rows not yet saturated), OR-ed into every row holding that column.
+ * Same results either way. Process-wide; default 1. */
+int bgb_set_bfs_gather_mode(int mode);
+int bgb_get_bfs_gather_mode(void);
+
 /* Measurement aid (no reference counterpart): with on = 1, calls of
  * bgb_bfs_multi / bgb_bfs_multi_dir with use_graph = 0 run one level at a
  * time between two HIP events (and read the level flags before each level,

@@ -72,6 +72,39 @@ def test_bfs_plain_matches_oracle(orc, dev, T):
         assert nl == wl
 
 
+@pytest.mark.parametrize("T", (4, 8))
+@pytest.mark.parametrize("mode", (0, 1))
+def test_bfs_gather_modes(orc, dev, T, mode):
+    """Both frontier-gather walks of the pull levels (per set bit of a row, per
+    set column of a tile with the group's first columns gathered ahead) give the
+    oracle's levels: hub rows that saturate early, a hub in the last block row,
+    isolated vertices and n mod T != 0."""
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(40 + T + mode)
+    n = 20003
+    src_e = rng.integers(0, n, 60000)
+    dst_e = rng.integers(0, n, 60000)
+    hub = np.concatenate([np.zeros(4000, np.int64), np.full(3000, n - 2)])
+    nb = rng.integers(0, n - 50, hub.size)
+    csr = graph.from_coo(n, n, np.concatenate([src_e, hub]), np.concatenate([dst_e, nb]))
+    A = B.B2SR.from_csr(csr, T, dev)
+    src = rng.integers(0, n, 64)
+    src[:2] = (0, n - 2)
+    want = oracle_levels(orc, csr, src)
+    prev = bgb.lib().bgb_get_bfs_gather_mode()
+    bgb.call("bgb_set_bfs_gather_mode", mode)
+    try:
+        for push in (None, A):
+            log(f"[case] gather mode {mode} T={T} push={push is not None}")
+            lev, nl = B.bfs_multi(A, src, push=push)
+            np.testing.assert_array_equal(lev.cpu().numpy(), want)
+            assert nl == want.max() + 1
+    finally:
+        bgb.call("bgb_set_bfs_gather_mode", prev)
+
+
 def test_bfs_rmat_batched_equals_plain_and_oracle(orc, dev):
     from bgb import b2sr as B
     from bgb import graph
