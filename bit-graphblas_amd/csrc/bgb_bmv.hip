@@ -293,12 +293,7 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
         }
         // rows 0, 2 and 1, 3 in 16-bit fields (a wave sum is at most 2048)
         auto wave_rows = [&](uint32_t pk, Acc (&t)[R]) {
-          uint32_t lo = pk & 0x00FF00FFu, hi = (pk >> 8) & 0x00FF00FFu;
-#pragma unroll
-          for (int m = 1; m < 64; m <<= 1) {
-            lo += (uint32_t)__shfl_xor((int)lo, m);
-            hi += (uint32_t)__shfl_xor((int)hi, m);
-          }
+          const uint32_t lo = wave_total(pk & 0x00FF00FFu), hi = wave_total((pk >> 8) & 0x00FF00FFu);
           t[0] = lo & 0xFFFFu, t[1] = hi & 0xFFFFu, t[2] = lo >> 16, t[3] = hi >> 16;
         };
         wave_rows(pka, ta);
@@ -317,10 +312,21 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
           }
         }
         auto wave_rows = [&](Acc (&t)[R]) {
+          if constexpr (L == 1 && !is_full(OP)) {
+            // two rows per register in 16-bit fields (a row's chunk total is
+            // at most CH x T <= 8192), summed over the wave with DPP
 #pragma unroll
-          for (int m = L; m < 64; m <<= 1)
+            for (int j = 0; j < R / 2; ++j) {
+              const uint32_t w = wave_total(t[2 * j] | (t[2 * j + 1] << 16));
+              t[2 * j] = w & 0xFFFFu;
+              t[2 * j + 1] = w >> 16;
+            }
+          } else {
 #pragma unroll
-            for (int i = 0; i < R; ++i) t[i] += (Acc)__shfl_xor((int)t[i], m);
+            for (int m = L; m < 64; m <<= 1)
+#pragma unroll
+              for (int i = 0; i < R; ++i) t[i] += (Acc)__shfl_xor((int)t[i], m);
+          }
         };
         wave_rows(ta);
         if (two) wave_rows(tb);
@@ -491,18 +497,40 @@ __device__ __forceinline__ void bmv_chunk(const BmvArgs &a, int chunk, int lane,
 
   // Segmented inclusive scan of the open-row partials across groups (keys
   // are non-decreasing over groups, so equal keys are contiguous).
+  // Count and bit ops scan two rows per register, in 16-bit fields: a row's
+  // count within one chunk is at most (tiles of the chunk) x T <= 8192, so
+  // the fields never carry into each other -- half the scan's DPP steps.
   const int key = active ? I : INT_MAX;
   Acc sc[R];
-#pragma unroll
-  for (int i = 0; i < R; ++i) sc[i] = acc[i];
-  group_scan<L, R, Acc>(key, sc, [](Acc x, Acc y) { return x + y; });
-  // Carry into this group's first flushed row from the groups before it.
-  const int kin = group_up<L>(key);
   Acc cin[R];
+  const int kin = group_up<L>(key);
+  const bool cin_on = g > 0 && kin == I0;
+  if constexpr (!is_full(OP)) {
+    static_assert(R % 2 == 0 && CH * T < 65536, "two 16-bit row counts per register");
+    uint32_t pk[R / 2];
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    const Acc up = group_up<L, Acc>(sc[i]);
-    cin[i] = (g > 0 && kin == I0) ? up : (Acc)0;
+    for (int j = 0; j < R / 2; ++j) pk[j] = acc[2 * j] | (acc[2 * j + 1] << 16);
+    group_scan<L, R / 2, uint32_t>(key, pk, [](uint32_t x, uint32_t y) { return x + y; });
+    // Carry into this group's first flushed row from the groups before it.
+#pragma unroll
+    for (int j = 0; j < R / 2; ++j) {
+      const uint32_t upv = group_up<L, uint32_t>(pk[j]);  // every lane takes part
+      const uint32_t up = cin_on ? upv : 0u;
+      sc[2 * j] = pk[j] & 0xFFFFu;
+      sc[2 * j + 1] = pk[j] >> 16;
+      cin[2 * j] = up & 0xFFFFu;
+      cin[2 * j + 1] = up >> 16;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i) sc[i] = acc[i];
+    group_scan<L, R, Acc>(key, sc, [](Acc x, Acc y) { return x + y; });
+    // Carry into this group's first flushed row from the groups before it.
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const Acc up = group_up<L, Acc>(sc[i]);
+      cin[i] = cin_on ? up : (Acc)0;
+    }
   }
   if (held >= 0) {
 #pragma unroll

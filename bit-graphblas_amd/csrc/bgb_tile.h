@@ -518,6 +518,20 @@ __device__ __forceinline__ uint64_t dpp_wide(uint64_t v, uint64_t old) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Sum of v over the wave, returned in every lane: DPP row_shr steps give each
+// lane its 16-lane row's inclusive prefix, the row broadcasts carry the row
+// totals up to lane 63, which is read out -- VALU only (a shuffle reduction
+// is six LDS round trips per value).
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+  v += dpp_shr<1>(v, 0u);
+  v += dpp_shr<2>(v, 0u);
+  v += dpp_shr<4>(v, 0u);
+  v += dpp_shr<8>(v, 0u);
+  v += dpp_wide<kBcast15>(v, 0u);
+  v += dpp_wide<kBcast31>(v, 0u);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Value of lane (lane - L) for lanes >= L (others get `old`): the previous
 // lane group's lane in the same position. DPP wave_shr:1 for L = 1,
 // ds_bpermute otherwise.
