@@ -48,6 +48,8 @@ SIGNATURES = {
     "bgb_set_x_staging": (_c_int, [_c_int, _c_int]),
     "bgb_get_x_staging": (_c_int, [_c_int]),
     "bgb_set_short_row_path": (_c_int, [_c_int]),
+    "bgb_set_row_pieces": (_c_int, [_c_int]),
+    "bgb_get_row_pieces": (_c_int, []),
     "bgb_get_short_row_path": (_c_int, []),
     "bgb_bmv_bin_bin_bin": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),
     "bgb_bmv_bin_bin_full_i32": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp]),

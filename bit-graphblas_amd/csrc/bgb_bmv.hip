@@ -76,6 +76,10 @@ int g_x_staging[5] = {0, 0, 0, 0, 0};
 // Short-row path (T <= 8): 0 automatic (every block row at most kShortRowMax
 // tiles), 1 never, 2 always.
 int g_short_rows = 0;
+// Row pieces for the count ops at T <= 8 (bgb_set_row_pieces): 0 automatic
+// (when the packed x fits in LDS and the pieces are mostly full), 1 never
+// (the chunk walk), 2 always.
+int g_row_pieces = 0;
 
 template <int OP>
 struct AccOf {
@@ -589,22 +593,11 @@ __device__ __forceinline__ int2 load_part_vec(const BmvArgs &a, int c) {
   return make_int2(ia, cv + 1 < a.nchunks ? ib : 0);
 }
 
-// Bit-vector ops with the whole packed x in LDS (it fits for n up to ~1.1M):
-// one workgroup of lds_waves<T, S>() waves per CU (or two, if they fit) stages x
-// once, then each wave walks chunks gw, gw + W, ... (W waves in the grid).
-// The per-tile x gathers become LDS reads instead of one L2 request each.
-// 16 waves (128 VGPRs each) hold two chunks' inputs for T <= 8 and run
-// length 8; the larger fragments need the 256 VGPRs of 8-wave workgroups.
-template <int T, int S>
-constexpr int lds_waves() { return (T >= 16 || S > 8) ? 8 : 16; }
-// XL = 1: all of x staged (x_bytes of it); XL = 2: its first a.xl_words words
-// (x_bytes bytes), the other words read from memory.
-template <int T, int OP, int S, int XL>
-__global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
-  constexpr int kLdsWaves = lds_waves<T, S>();
-  extern __shared__ __align__(16) uint8_t s_dyn[];
-  int *win_all = reinterpret_cast<int *>(s_dyn);
-  uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWinAlloc;
+// Copies the packed x (x_bytes bytes of a.B, 16-byte aligned) into LDS at xl,
+// all threads of the workgroup; the caller synchronises. T = 4 keeps two
+// words (low nibbles) per LDS byte.
+template <int T>
+__device__ __forceinline__ void stage_x(const BmvArgs &a, int x_bytes, uint8_t *xl) {
   // stage the packed x (x_bytes bytes, 16-byte aligned): whole 16-byte
   // units, then the tail byte by byte
   const uint4 *src = static_cast<const uint4 *>(a.B);
@@ -630,6 +623,25 @@ __global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvAr
     for (int i = threadIdx.x; i < units; i += blockDim.x) dst[i] = src[i];
     for (int k = units * 16 + threadIdx.x; k < x_bytes; k += blockDim.x) xl[k] = srcb[k];
   }
+}
+
+// Bit-vector ops with the whole packed x in LDS (it fits for n up to ~1.1M):
+// one workgroup of lds_waves<T, S>() waves per CU (or two, if they fit) stages x
+// once, then each wave walks chunks gw, gw + W, ... (W waves in the grid).
+// The per-tile x gathers become LDS reads instead of one L2 request each.
+// 16 waves (128 VGPRs each) hold two chunks' inputs for T <= 8 and run
+// length 8; the larger fragments need the 256 VGPRs of 8-wave workgroups.
+template <int T, int S>
+constexpr int lds_waves() { return (T >= 16 || S > 8) ? 8 : 16; }
+// XL = 1: all of x staged (x_bytes of it); XL = 2: its first a.xl_words words
+// (x_bytes bytes), the other words read from memory.
+template <int T, int OP, int S, int XL>
+__global__ __launch_bounds__((64 * lds_waves<T, S>())) void bmv_lds_kernel(BmvArgs a, int x_bytes) {
+  constexpr int kLdsWaves = lds_waves<T, S>();
+  extern __shared__ __align__(16) uint8_t s_dyn[];
+  int *win_all = reinterpret_cast<int *>(s_dyn);
+  uint8_t *xl = s_dyn + sizeof(int) * kLdsWaves * kWinAlloc;
+  stage_x<T>(a, x_bytes, xl);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = gridDim.x * kLdsWaves;
@@ -874,6 +886,300 @@ __global__ __launch_bounds__(256) void bmv_rows_kernel(BmvArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row pieces (count ops, T <= 8). The chunk walk cuts the tile array into
+// fixed chunks regardless of rows, and pays for it per chunk: the group-row
+// pass, a row-end test per tile, the segmented scan and carries. Here the
+// work is cut along rows instead, classified once per matrix:
+//   * short block rows (at most kShortRowMax tiles): one lane per row
+//     (block_row_sums), as in the short-row path;
+//   * longer rows: pieces of piece_tiles<T>() tiles on a grid aligned to the row's
+//     first 16-byte unit of tile words, one wave per piece: lanes stream
+//     whole 16-byte units (4 tiles at T = 4, 2 at T = 8), count with
+//     byte-wise popcounts, and one DPP wave sum gives the piece's row totals
+//     -- no row ends inside a piece. A row's last piece writes the row (plain
+//     store or accumulate); its other pieces leave carry records, which the
+//     chunk path's fix-up adds (records of one row are consecutive).
+// Tiles outside the piece (the aligned unit's head, loads past the matrix)
+// count nothing: colind and the tile words are read through buffer
+// resources, which return zeros out of range.
+// Tiles per piece: 256 at T = 4 (one 16-byte unit per lane), 256 at T = 8
+// (two units per lane).
+template <int T>
+constexpr int piece_tiles() { return 256; }
+
+// Pieces of block row I (0: a short row, taken by the one-lane-per-row pass).
+template <int T>
+__device__ __forceinline__ int row_pieces(int s, int e) {
+  constexpr int A = T == 4 ? 4 : 2;  // tiles per 16-byte unit
+  return e - s > kShortRowMax ? (e - (s & ~(A - 1)) + piece_tiles<T>() - 1) / piece_tiles<T>() : 0;
+}
+
+template <int T>
+__global__ void piece_count_kernel(const int *__restrict__ rowptr, int nbr, int *__restrict__ np) {
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < nbr) np[I] = row_pieces<T>(rowptr[I], rowptr[I + 1]);
+}
+
+// Piece descriptors: {row, 2 * (piece of the row) + (last piece of the row),
+// the row's first and end tile when the plan was made}.
+__global__ void piece_fill_kernel(const int *__restrict__ rowptr, const int *__restrict__ np,
+                                  const int *__restrict__ off, int nbr, int4 *__restrict__ pieces) {
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const int n = np[I], o = off[I], s = rowptr[I], e = rowptr[I + 1];
+  for (int j = 0; j < n; ++j) pieces[o + j] = make_int4((int)I, 2 * j + (j == n - 1 ? 1 : 0), s, e);
+}
+
+// The short rows of a piece plan (np[I] == 0), one lane per block row; empty
+// rows are left to the fix-up's zeroing pass.
+template <int T, int OP>
+__global__ __launch_bounds__(256) void bmv_short_rows_kernel(BmvArgs a, const int *__restrict__ np) {
+  using Acc = typename AccOf<OP>::type;
+  for (long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x; I < a.nbr;
+       I += (long long)gridDim.x * blockDim.x) {
+    const int s = a.rowptr[I], e = a.rowptr[I + 1];
+    if (np[I] != 0 || s == e) continue;
+    Acc acc[Tile<T>::R];
+    block_row_sums<T, kRowCount, 8>(a.A, a.colind, a.B, s, e, acc);
+    flush_row<T, OP>(a, (int)I, 0, acc);
+  }
+}
+
+// Byte-wise popcounts of a word: byte r of the result = bits set in byte r.
+__device__ __forceinline__ uint32_t byte_popc(uint32_t m) {
+  m = m - ((m >> 1) & 0x55555555u);
+  m = (m & 0x33333333u) + ((m >> 2) & 0x33333333u);
+  return (m + (m >> 4)) & 0x0F0F0F0Fu;
+}
+
+// One wave per piece, persistent over the piece list (waves take pieces
+// w, w + nw, ...). XL: packed x in LDS -- 1 all of it, 2 its first
+// a.xl_words words (the rest from memory), 0 none (gathered from memory).
+// A piece's inputs: its tile words and block columns (one 16-byte unit of
+// tile words per lane and step; units outside the piece read nothing, their
+// offsets lying past the buffers), the row's extent now, and the packed-x
+// word of each tile (0 outside the piece).
+template <int T>
+struct PieceIn {
+  static constexpr int A = T == 4 ? 4 : 2;          // tiles per 16-byte unit
+  static constexpr int STEPS = piece_tiles<T>() / (64 * A);  // units per lane
+  uint4 wv[STEPS];
+  uint32_t jv[STEPS][A];
+  uint32_t xw[STEPS][A];
+  int s_now, e_now;
+};
+
+template <int T>
+__device__ __forceinline__ int piece_base(int s, int jl) {
+  return (s & ~(PieceIn<T>::A - 1)) + (jl >> 1) * piece_tiles<T>();
+}
+// End of the tiles piece d covers (a row's last piece: the row's end).
+template <int T>
+__device__ __forceinline__ int piece_hi(int4 d, int base) {
+  return (d.y & 1) ? d.w : min(base + piece_tiles<T>(), d.w);
+}
+
+// Issues the tile-word, block-column and row-extent loads of piece d.
+template <int T>
+__device__ __forceinline__ void piece_load(const BmvArgs &a, __amdgpu_buffer_rsrc_t vr, __amdgpu_buffer_rsrc_t cr,
+                                           int4 d, int lane, PieceIn<T> &in) {
+  constexpr int A = PieceIn<T>::A, TB = T == 4 ? 4 : 8;
+  const int base = piece_base<T>(d.z, d.y);
+  const int hi = piece_hi<T>(d, base);
+#pragma unroll
+  for (int st = 0; st < PieceIn<T>::STEPS; ++st) {
+    const int t0 = base + (lane + 64 * st) * A;
+    const bool need = t0 < hi;
+    in.wv[st] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                              vr, need ? t0 * TB : a.nblocks * TB, 0, 0));
+    if constexpr (A == 4) {
+      const uint4 c = __builtin_bit_cast(
+          uint4, __builtin_amdgcn_raw_buffer_load_b128(cr, need ? t0 * 4 : a.nblocks * 4, 0, 0));
+      in.jv[st][0] = c.x, in.jv[st][1] = c.y, in.jv[st][2] = c.z, in.jv[st][3] = c.w;
+    } else {
+      const uint2 c = __builtin_bit_cast(
+          uint2, __builtin_amdgcn_raw_buffer_load_b64(cr, need ? t0 * 4 : a.nblocks * 4, 0, 0));
+      in.jv[st][0] = c.x, in.jv[st][1] = c.y;
+    }
+  }
+  // vector loads (a scalar load's wait would hold up every LDS read)
+  int iv = d.x;
+  asm volatile("" : "+v"(iv));
+  in.s_now = a.rowptr[iv];
+  in.e_now = a.rowptr[iv + 1];
+}
+
+// Issues the packed-x reads of piece d's tiles (its block columns loaded):
+// XL 1 all of x staged in LDS, 2 its first a.xl_words words (the others
+// from memory), 0 none.
+template <int T, int XL>
+__device__ __forceinline__ void piece_xload(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr, int4 d,
+                                            int lane, PieceIn<T> &in) {
+  using W = typename Tile<T>::W;
+  constexpr int A = PieceIn<T>::A;
+  const int base = piece_base<T>(d.z, d.y);
+  const int lo = base > d.z ? base : d.z, hi = piece_hi<T>(d, base);
+#pragma unroll
+  for (int st = 0; st < PieceIn<T>::STEPS; ++st)
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      const int tt = base + (lane + 64 * st) * A + u;
+      const bool inr = tt >= lo && tt < hi;
+      const int J = inr ? (int)in.jv[st][u] : 0;
+      uint32_t x;
+      if constexpr (XL == 1) {
+        x = lds_xword<T>(xl, J);
+      } else if constexpr (XL == 2) {
+        const bool staged = !inr || J < a.xl_words;
+        const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
+        const int off = staged ? a.x_words * (int)sizeof(W) : J * (int)sizeof(W);
+        const uint32_t xm = __builtin_amdgcn_raw_buffer_load_b8(xr, off, 0, 0);
+        x = staged ? xs : xm;
+      } else {
+        x = __builtin_amdgcn_raw_buffer_load_b8(xr, inr ? J * (int)sizeof(W) : a.x_words * (int)sizeof(W), 0, 0);
+      }
+      in.xw[st][u] = inr ? x : 0u;
+    }
+}
+
+// Row totals t[] += the counts of a loaded piece (tiles outside it have x 0).
+template <int T>
+__device__ __forceinline__ void piece_count(const PieceIn<T> &in, uint32_t (&t)[T]) {
+  constexpr int A = PieceIn<T>::A;
+  uint32_t pk[T / 4];
+#pragma unroll
+  for (int q = 0; q < T / 4; ++q) pk[q] = 0;
+#pragma unroll
+  for (int st = 0; st < PieceIn<T>::STEPS; ++st) {
+    const uint32_t w4[4] = {in.wv[st].x, in.wv[st].y, in.wv[st].z, in.wv[st].w};
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      const uint32_t xr4 = in.xw[st][u] * 0x01010101u;
+      // byte-wise row counts (a lane adds at most piece_tiles / 64 tiles of <= T each)
+#pragma unroll
+      for (int q = 0; q < T / 4; ++q) pk[q] += byte_popc(w4[u * (T / 4) + q] & xr4);
+    }
+  }
+  // wave totals, two rows per register in 16-bit fields (at most piece_tiles * T)
+#pragma unroll
+  for (int q = 0; q < T / 4; ++q) {
+    const uint32_t ev = wave_total(pk[q] & 0x00FF00FFu);         // rows 4q, 4q + 2
+    const uint32_t od = wave_total((pk[q] >> 8) & 0x00FF00FFu);  // rows 4q + 1, 4q + 3
+    t[4 * q] += ev & 0xFFFFu;
+    t[4 * q + 2] += ev >> 16;
+    t[4 * q + 1] += od & 0xFFFFu;
+    t[4 * q + 3] += od >> 16;
+  }
+}
+
+// One wave per piece, persistent over the piece list (waves take pieces
+// w, w + nw, ...), software-pipelined three deep: while piece p is counted,
+// the x words of the wave's next piece and the tile words, block columns
+// and row extent of the one after are in flight. XL: packed x in LDS -- 1
+// all of it, 2 its first a.xl_words words (the rest from memory), 0 none.
+template <int T, int OP, int XL>
+__global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes, const int4 *__restrict__ pieces,
+                                                         int npieces) {
+  static_assert(T == 4 || T == 8, "one lane per tile");
+  using W = typename Tile<T>::W;
+  using Acc = typename AccOf<OP>::type;
+  static_assert(std::is_same<Acc, uint32_t>::value, "count ops");
+  constexpr int TB = T == 4 ? 4 : 8;  // bytes of tile words per tile
+  extern __shared__ __align__(16) uint8_t s_dyn[];
+  uint8_t *xl = s_dyn;
+  if constexpr (XL != 0) {
+    stage_x<T>(a, x_bytes, xl);
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const int nw = (int)gridDim.x * ((int)blockDim.x >> 6);
+  int p = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6));
+  if (p >= npieces) return;  // uniform per wave; no barrier follows
+  const int p0 = p;
+  const __amdgpu_buffer_rsrc_t cr =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(a.colind), (short)0, a.nblocks * 4, 0x00020000);
+  const __amdgpu_buffer_rsrc_t vr =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.A), (short)0, a.nblocks * TB, 0x00020000);
+  __amdgpu_buffer_rsrc_t xr;
+  if constexpr (XL != 1)
+    xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.B), (short)0, a.x_words * (int)sizeof(W),
+                                           0x00020000);
+  // descriptors as vector loads; past the wave's last piece every load
+  // (descriptor, tiles, x) repeats its first piece, results unused, so that
+  // all loads stay unconditional and the waits count them exactly
+  auto load_desc = [&](int pp) {
+    int pv = pp < npieces ? pp : p0;
+    asm volatile("" : "+v"(pv));
+    return pieces[pv];
+  };
+  auto uni = [](int4 d) {
+    return make_int4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
+                     __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
+  };
+  // d0: piece p (x in flight or arrived), d1: p + nw (tiles in flight),
+  // dn: the descriptor of p + 2nw (in flight)
+  int4 d0 = uni(load_desc(p));
+  int4 d1 = uni(load_desc(p + nw));
+  int4 dn = load_desc(p + 2 * nw);
+  auto step = [&](PieceIn<T> &cur, PieceIn<T> &nxt, PieceIn<T> &far) {
+    const int pc = p;
+    const bool more = p + nw < npieces;  // wave-uniform
+    const int4 d2 = uni(dn);
+    piece_load<T>(a, vr, cr, d2, lane, far);   // tiles of p + 2nw
+    dn = load_desc(p + 3 * nw);
+    piece_xload<T, XL>(a, xl, xr, d1, lane, nxt);  // x of p + nw
+    const int I = d0.x, jl = d0.y;
+    const bool last = (jl & 1) != 0;
+    uint32_t t[T];
+#pragma unroll
+    for (int r = 0; r < T; ++r) t[r] = 0;
+    const int s_now = __builtin_amdgcn_readfirstlane(cur.s_now), e_now = __builtin_amdgcn_readfirstlane(cur.e_now);
+    if (s_now == d0.z && e_now == d0.w) {
+      piece_count<T>(cur, t);
+    } else {
+      // rowptr was rewritten since the plan: count the row as it is now, a
+      // window at a time (the row's last piece runs on to the row's end)
+      const int base0 = piece_base<T>(s_now, jl);
+      constexpr int P = piece_tiles<T>();
+      const int kend = last ? e_now : min(base0 + P, e_now);
+      for (int kb = base0; kb < kend; kb += P) {
+        PieceIn<T> w;
+        const int4 dw = make_int4(I, 1, kb, min(kb + P, kend));  // one window, from kb
+        piece_load<T>(a, vr, cr, dw, lane, w);
+        piece_xload<T, XL>(a, xl, xr, make_int4(I, 1, kb > s_now ? kb : s_now, dw.w), lane, w);
+        piece_count<T>(w, t);
+      }
+      // drain everything here, so that after the join the compiler still
+      // counts the prefetches of the common path exactly (rather than
+      // waiting for all of them before the next piece)
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    if (lane == 0) {
+      if (last) {
+        flush_row<T, OP>(a, I, 0, t);
+        a.carry_key[pc] = -1;
+      } else {
+        a.carry_key[pc] = I;
+        typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+        U4 *cv = reinterpret_cast<U4 *>(static_cast<uint32_t *>(a.carry_val) + (long long)pc * T);
+#pragma unroll
+        for (int r = 0; r < T / 4; ++r) cv[r] = U4{t[4 * r], t[4 * r + 1], t[4 * r + 2], t[4 * r + 3]};
+      }
+    }
+    d0 = d1;
+    d1 = d2;
+    p += nw;
+    return more;
+  };
+  PieceIn<T> b0, b1, b2;
+  piece_load<T>(a, vr, cr, d0, lane, b0);
+  piece_load<T>(a, vr, cr, d1, lane, b1);
+  piece_xload<T, XL>(a, xl, xr, d0, lane, b0);
+  while (step(b0, b1, b2) && step(b1, b2, b0) && step(b2, b0, b1)) {
+  }
+}
+
 // Most tiles in one block row (device-wide max of rowptr[I+1] - rowptr[I]):
 // a grid-stride loop over at most kMaxRowBlocks workgroups of 256, a
 // workgroup max, one atomic per workgroup (one per wave on one address
@@ -962,6 +1268,138 @@ int static_scratch(uint8_t **out) {
   return BGB_OK;
 }
 
+// Piece plans of the row-piece path, one per (device, rowptr address, nbr,
+// nblocks, T): per block row its number of pieces (0: short), and the piece
+// descriptors. Made on the first call for a matrix (one read-back of the
+// piece count, which synchronises the stream) and kept for the next. The
+// kernels read the row extents from rowptr itself, and a row's last piece
+// runs on to the row's end, so a rowptr rewritten in place (same nbr) still
+// gives exact results from a stale plan.
+namespace {
+struct PieceEntry {
+  int dev;
+  const int *rowptr;
+  int nbr, nblocks, T;
+  int *np;        // nbr + 1 ints
+  int4 *pieces;   // npieces
+  int npieces;
+};
+constexpr int kPieceCache = 8;
+std::mutex g_piece_mu;
+std::vector<PieceEntry> g_piece;  // most recently used last
+
+void free_piece(const PieceEntry &e) {
+  (void)hipSetDevice(e.dev);
+  (void)hipDeviceSynchronize();  // queued work on any stream may still read it
+  (void)hipFree(e.np);
+  (void)hipFree(e.pieces);
+}
+}  // namespace
+
+template <int T>
+int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntry *out) {
+  int dev = 0;
+  BGB_HIP_TRY(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_piece_mu);
+    for (size_t i = 0; i < g_piece.size(); ++i) {
+      const PieceEntry e = g_piece[i];
+      if (e.dev == dev && e.rowptr == rowptr && e.nbr == nbr && e.nblocks == nblocks && e.T == T) {
+        g_piece.erase(g_piece.begin() + (long)i);
+        g_piece.push_back(e);
+        *out = e;
+        return BGB_OK;
+      }
+    }
+  }
+  PieceEntry ne{dev, rowptr, nbr, nblocks, T, nullptr, nullptr, 0};
+  DeviceBuffer off;
+  hipError_t he = hipMalloc(&ne.np, sizeof(int) * ((size_t)nbr + 1));
+  if (he != hipSuccess) return set_error(BGB_ERR_ALLOC, "piece plan hipMalloc failed: %s", hipGetErrorString(he));
+  auto fail = [&](int code, const char *what, hipError_t err) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(ne.np);
+    if (ne.pieces) (void)hipFree(ne.pieces);
+    return set_error(code, "piece plan: %s: %s", what, hipGetErrorString(err));
+  };
+  if ((he = off.alloc(sizeof(int) * ((size_t)nbr + 1), s)) != hipSuccess) return fail(BGB_ERR_ALLOC, "alloc", he);
+  fill_kernel<int><<<1, 64, 0, s>>>(ne.np + nbr, 1, 0);
+  piece_count_kernel<T><<<(unsigned)ceil_div(nbr, 256), 256, 0, s>>>(rowptr, nbr, ne.np);
+  if ((he = hipGetLastError()) != hipSuccess) return fail(BGB_ERR_HIP, "count", he);
+  if ((he = exclusive_scan_i32(ne.np, off.as<int>(), (size_t)nbr + 1, s)) != hipSuccess)
+    return fail(BGB_ERR_HIP, "scan", he);
+  int total = 0;
+  if ((he = hipMemcpyAsync(&total, off.as<int>() + nbr, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (he = hipStreamSynchronize(s)) != hipSuccess)
+    return fail(BGB_ERR_HIP, "read-back", he);
+  ne.npieces = total;
+  if ((he = hipMalloc(&ne.pieces, sizeof(int4) * (size_t)std::max(total, 1))) != hipSuccess)
+    return fail(BGB_ERR_ALLOC, "hipMalloc", he);
+  if (total > 0) {
+    piece_fill_kernel<<<(unsigned)ceil_div(nbr, 256), 256, 0, s>>>(rowptr, ne.np, off.as<int>(), nbr, ne.pieces);
+    if ((he = hipGetLastError()) != hipSuccess) return fail(BGB_ERR_HIP, "fill", he);
+  }
+  std::lock_guard<std::mutex> lk(g_piece_mu);
+  if ((int)g_piece.size() >= kPieceCache) {
+    free_piece(g_piece.front());
+    g_piece.erase(g_piece.begin());
+    (void)hipSetDevice(dev);
+  }
+  g_piece.push_back(ne);
+  *out = ne;
+  return BGB_OK;
+}
+
+// The row-piece path's main passes: the short rows (one lane per block row)
+// and the pieces (persistent waves, the packed x staged in LDS when it fits,
+// or its first words when a useful share of it does, else read from memory).
+template <int T, int OP>
+int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
+  using W = typename Tile<T>::W;
+  DeviceFacts f;
+  int st = device_facts(&f);
+  if (st) return st;
+  // the largest block column sizes x (the range of its buffer resource and of
+  // the staged words): a wide matrix reads past ceil(NBR/T)*T words
+  int maxcol = -1;
+  const int mode = g_x_staging[ilog2_tiledim(T) - 2];
+  if ((st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
+  BGB_REQUIRE(maxcol >= 0 || a.nblocks == 0, BGB_ERR_ARG, "largest block column unknown (stream capture?)");
+  const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
+  const long long x_bytes = words * (long long)sizeof(W);
+  const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;  // LDS bytes of all of x
+  const long long lds_max = f.lds_per_block & ~15ll;
+  const long long part_words = T == 4 ? 2 * lds_max : lds_max / (long long)sizeof(W);
+  const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
+  const bool fits = mode != 1 && aligned && xl <= lds_max;
+  const bool partial = mode != 1 && aligned && !fits && x_bytes < (1ll << 31) &&
+                       (mode == 2 || 8 * part_words >= words);
+  BGB_REQUIRE(x_bytes < (1ll << 31), BGB_ERR_SIZE, "packed x of %lld bytes past a buffer resource's range",
+              x_bytes);
+  a.x_words = (int)words;
+  a.xl_words = partial ? (int)part_words : 0;
+  bmv_short_rows_kernel<T, OP><<<capped_blocks(a.nbr, 256), 256, 0, s>>>(a, pe.np);
+  BGB_HIP_TRY(hipGetLastError());
+  if (pe.npieces == 0) return BGB_OK;
+  const int xlk = fits ? 1 : partial ? 2 : 0;
+  const long long lds = xlk == 1 ? xl : xlk == 2 ? lds_max : 0;
+  void (*k)(BmvArgs, int, const int4 *, int) = xlk == 1   ? &bmv_piece_kernel<T, OP, 1>
+                                              : xlk == 2 ? &bmv_piece_kernel<T, OP, 2>
+                                                         : &bmv_piece_kernel<T, OP, 0>;
+  if (lds > 0)
+    BGB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  int per_cu = 0;
+  BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), 1024,
+                                                           (size_t)lds));
+  const unsigned grid = (unsigned)std::min<long long>((long long)f.cus * std::max(per_cu, 1),
+                                                      ceil_div(pe.npieces, 16));
+  const int stage_bytes = xlk == 1 ? (int)x_bytes : xlk == 2 ? (int)(part_words * (long long)sizeof(W)) : 0;
+  k<<<grid, 1024, (size_t)lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces);
+  BGB_HIP_TRY(hipGetLastError());
+  return BGB_OK;
+}
+
 // scr: the static scratch of the allocation-free entry point (carry records
 // and a partition made on every call, x gathered from memory), or null for
 // the per-stream workspace, the cached partition and the automatic paths.
@@ -969,14 +1407,49 @@ template <int T, int OP, int S>
 int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   constexpr int CH = (64 / Tile<T>::L) * S;
   using Acc = typename AccOf<OP>::type;
-  const long long nchunks = ceil_div(a.nblocks, CH);
+  int st = 0;
+  // Count ops at T <= 8 take the row-piece path (not with the static scratch:
+  // its plan is cached and made with a read-back).
+  constexpr bool kPieceOp = (T == 4 || T == 8) && (OP == OP_COUNT_I32 || OP == OP_COUNT_F32);
+  PieceEntry pe{};
+  bool pieces = false;
+  if constexpr (kPieceOp) {
+    if (!scr && g_row_pieces != 1 && (long long)a.nblocks * 8 < (1ll << 31)) {
+      // automatic: when the whole packed x fits in LDS (with x partly in
+      // memory the pieces' per-slot x reads cost more than the walk saves:
+      // R-MAT 22, T = 4, 405 vs 387 us) and the pieces are mostly full
+      bool want = g_row_pieces == 2;
+      if (!want && g_x_staging[ilog2_tiledim(T) - 2] != 1) {
+        int maxcol = -1;
+        DeviceFacts f;
+        if ((st = device_facts(&f)) || (st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
+        using W = typename Tile<T>::W;
+        const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
+        const long long xb = words * (long long)sizeof(W);
+        const long long xl = ((T == 4 ? (xb + 1) / 2 : xb) + 15) & ~15ll;
+        want = maxcol >= 0 && (reinterpret_cast<uintptr_t>(a.B) & 15) == 0 && xl <= (f.lds_per_block & ~15ll);
+      }
+      if (want) {
+        if ((st = piece_plan<T>(s, a.rowptr, a.nbr, a.nblocks, &pe))) return st;
+        // tiles per piece slot (short rows' tiles included: a bound)
+        const double fill = (double)a.nblocks / ((double)std::max(pe.npieces, 1) * piece_tiles<T>());
+        pieces = g_row_pieces == 2 || fill >= 0.75;
+      }
+    }
+  }
+  const long long nchunks = pieces ? (long long)pe.npieces : ceil_div(a.nblocks, CH);
   BGB_REQUIRE(nchunks < INT_MAX / 2, BGB_ERR_SIZE, "too many chunks");
   a.nchunks = (int)nchunks;
   void *ws = nullptr;
-  const size_t key_bytes = (sizeof(int) * (size_t)nchunks + 255) & ~(size_t)255;
-  const size_t val_bytes = (sizeof(Acc) * (size_t)nchunks * T + 255) & ~(size_t)255;
-  int st = 0;
-  if (scr) {
+  const size_t key_bytes = (sizeof(int) * (size_t)std::max(nchunks, 1ll) + 255) & ~(size_t)255;
+  const size_t val_bytes = (sizeof(Acc) * (size_t)std::max(nchunks, 1ll) * T + 255) & ~(size_t)255;
+  if (pieces) {
+    if ((st = stream_workspace(s, key_bytes + val_bytes, &ws))) return st;
+    a.carry_key = static_cast<int *>(ws);
+    a.carry_val = static_cast<char *>(ws) + key_bytes;
+    if constexpr (kPieceOp)
+      if ((st = launch_pieces<T, OP>(a, s, pe))) return st;
+  } else if (scr) {
     const size_t need = 2 * key_bytes + val_bytes;  // keys, partition, records
     BGB_REQUIRE(need <= kScratchBytes, BGB_ERR_SIZE,
                 "%lld tiles at T=%d need %zu bytes of scratch; the static scratch holds %zu", (long long)a.nblocks,
@@ -992,8 +1465,9 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
     a.carry_val = static_cast<char *>(ws) + key_bytes;
     if ((st = chunk_partition(s, a.rowptr, a.nbr, a.nblocks, CH, &a.part))) return st;
   }
-  bool lds_path = false;
+  bool lds_path = pieces;
   if constexpr (!is_full(OP)) {
+    if (!pieces) {
     // Packed x in LDS when the tile stream is large next to staging it once
     // per workgroup: all of it when it fits, else its first words. The words
     // cover every block column the matrix references (a wide matrix reads
@@ -1053,6 +1527,7 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
         lds_path = true;
       }
     }
+    }
   }
   if (!lds_path) {
     const unsigned blocks = (unsigned)ceil_div(nchunks, kWavesPerBlock);
@@ -1066,6 +1541,7 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   a.fixup_blocks = (int)fix_blocks;
   // overwriting calls also zero the empty block rows (accumulate leaves them)
   const long long zero_blocks = (a.accumulate && OP != OP_MASKED) ? 0 : std::min<long long>(ceil_div(a.nbr, 256), 1024);
+  if (fix_blocks + zero_blocks == 0) return BGB_OK;  // no records (all rows short) and nothing to zero
   bmv_fixup_kernel<T, OP><<<(unsigned)(fix_blocks + zero_blocks), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
@@ -1403,6 +1879,15 @@ int bgb_set_run_length(int T, int S) {
 }
 
 int bgb_get_run_length(int T) { return valid_tiledim(T) ? g_run_length[ilog2_tiledim(T) - 2] : -1; }
+
+int bgb_set_row_pieces(int mode) {
+  clear_error();
+  BGB_REQUIRE(mode >= 0 && mode <= 2, BGB_ERR_ARG, "row-piece mode must be 0, 1 or 2 (got %d)", mode);
+  g_row_pieces = mode;
+  return BGB_OK;
+}
+
+int bgb_get_row_pieces(void) { return g_row_pieces; }
 
 int bgb_set_x_staging(int T, int mode) {
   clear_error();

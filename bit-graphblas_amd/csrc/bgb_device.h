@@ -37,6 +37,7 @@ hipError_t radix_sort_u64(const uint64_t *in, uint64_t *out, size_t n, int end_b
 hipError_t radix_sort_u64_bits(const uint64_t *in, uint64_t *out, size_t n, int begin_bit, int end_bit,
                                hipStream_t s);
 hipError_t exclusive_scan_u8_to_i64(const uint8_t *in, long long *out, size_t n, hipStream_t s);
+hipError_t exclusive_scan_i32(const int *in, int *out, size_t n, hipStream_t s);
 
 // Per-(device, stream) grow-only scratch used by the BMV carry records. The
 // pointer stays valid until a later call on the same stream asks for more.

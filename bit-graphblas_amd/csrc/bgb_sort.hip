@@ -41,4 +41,13 @@ hipError_t exclusive_scan_u8_to_i64(const uint8_t *in, long long *out, size_t n,
   return rocprim::exclusive_scan(tmp.ptr, bytes, in, out, 0LL, n, rocprim::plus<long long>(), s);
 }
 
+hipError_t exclusive_scan_i32(const int *in, int *out, size_t n, hipStream_t s) {
+  size_t bytes = 0;
+  hipError_t e = rocprim::exclusive_scan(nullptr, bytes, in, out, 0, n, rocprim::plus<int>(), s);
+  if (e != hipSuccess) return e;
+  DeviceBuffer tmp;
+  if ((e = tmp.alloc(bytes, s)) != hipSuccess) return e;
+  return rocprim::exclusive_scan(tmp.ptr, bytes, in, out, 0, n, rocprim::plus<int>(), s);
+}
+
 }  // namespace bgb

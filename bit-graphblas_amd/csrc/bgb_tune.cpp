@@ -133,6 +133,9 @@ const PathFit kFpChunk = {12.0, {6.0, 7.7, 27.0, 52.0}};
 // Bits from memory past the staged first words of a packed x larger than
 // LDS (R-MAT 22: about 55 % of the tiles' block columns staged).
 const PathFit kBitPart = {15.0, {3.25, 3.7, 8.3, 0}};
+// bin_bin_full at T <= 8 with the whole packed x in LDS: the row-piece path
+// (R-MAT 20: T = 4 68.8 us, T = 8 80.3 us).
+const PathFit kBitPiece = {8.0, {2.24, 3.05, 0, 0}};
 
 // Packed-x placement of the bit op: 0 gathered from memory, 1 all of it in
 // LDS, 2 its first words in LDS (the launch rule of the BMV kernels: T <= 16
@@ -162,7 +165,7 @@ double predict_us(int op, int t, int S, const bgb_tile_profile &p, int *x_stagin
     const PathFit *f;
     if (short_rows) f = op == 0 ? &kBitShort : &kFpShort;
     else if (op == 0 && lds_path(T, p)) {
-      f = lds_path(T, p) == 1 ? &kBitLds : &kBitPart;
+      f = lds_path(T, p) == 1 ? (T <= 8 ? &kBitPiece : &kBitLds) : &kBitPart;
       if (x_staging) *x_staging = 2;
     } else f = op == 0 ? &kBitMem : &kFpChunk;
     double a = f->per_mtile[t];

@@ -155,6 +155,21 @@ int bgb_get_x_staging(int T);
 int bgb_set_short_row_path(int mode);
 int bgb_get_short_row_path(void);
 
+/* Row pieces for bgb_bmv_bin_bin_full_i32 / _f32 at T = 4 and 8: block rows of
+ * at most 32 tiles are counted by one lane each, longer rows in pieces of 256
+ * tiles by one wave each (whole 16-byte units per lane, a DPP wave sum per
+ * piece), with no row ends inside a piece; the pieces that do not end a row
+ * leave carry records for the fix-up. The plan (pieces per block row) is
+ * made on the first call for a (rowptr, nblockrows, nblocks, T) -- a count,
+ * a scan and one stream synchronisation -- and kept; the kernels read the
+ * row extents from rowptr itself, so a rowptr rewritten in place still gives
+ * exact results. 0 = automatic: taken when the whole packed x fits in LDS
+ * and the pieces are mostly full (otherwise the chunk walk); 1 = never,
+ * 2 = always. Process-wide tuning knob (no reference counterpart). Default 0.
+ * Same results on either path. */
+int bgb_set_row_pieces(int mode);
+int bgb_get_row_pieces(void);
+
 /* C[I] (|)= bits(T-1-r) = OR_c A[I*T+r][c] & x[c].  bmv{T}_bin_bin_bin
  * (bsrbmv.cu:277,437,508,579). B: packed bit vector, C: W_T[NBR]. */
 int bgb_bmv_bin_bin_bin(int T, const void *d_A, const void *d_B, void *d_C,

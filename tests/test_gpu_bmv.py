@@ -259,6 +259,52 @@ def _rmat20():
     return _RMAT20
 
 
+@pytest.mark.parametrize("T", (4, 8))
+def test_row_pieces_modes(orc, dev, T):
+    """The T <= 8 count BMVs cut long block rows into 512-tile pieces aligned to
+    the row's first 16-byte unit (one wave each, the last piece writing the row,
+    the others leaving carry records) and give short rows a lane each. The
+    piece path and the chunk walk must both give the oracle's counts: rows of
+    31..34 tiles (the short/long threshold), 511..513 and 1025 tiles (piece
+    edges), row starts at every alignment, an R-MAT graph, accumulate=1."""
+    import torch
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(70 + T)
+    n = 12000 * T // 4 + 3
+    lens = [31, 32, 33, 34, 1, 511, 512, 513, 2, 1025, 3, 700, 0, 5, 64, 65, 40]
+    src, dst = [], []
+    for i, k in enumerate(lens):
+        cols = rng.choice(n // T, k, replace=False) * T + rng.integers(0, T, k)
+        src.append(np.full(k, (i * 3 + 1) * T + i % T))
+        dst.append(cols)
+    src.append(rng.integers(0, n, 20000))
+    dst.append(rng.integers(0, n, 20000))
+    cases = (("piece edges", graph.from_coo(n, n, np.concatenate(src), np.concatenate(dst), False, False)),
+             ("rmat14", graph.rmat(14, 8, seed=31)))
+    prev = bgb.lib().bgb_get_row_pieces()
+    try:
+        for mode in (2, 1):
+            bgb.call("bgb_set_row_pieces", mode)
+            for label, csr in cases:
+                log(f"[case] row pieces mode {mode} {label} T={T}")
+                A = B.B2SR.from_csr(csr, T, dev)
+                x01 = (rng.random(csr.nrows) < 0.4).astype(np.float32)
+                xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+                want = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+                nbr = A.nblockrows
+                got = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(nbr * T, torch.int32, dev)))
+                np.testing.assert_array_equal(got[: csr.nrows], want)
+                assert not got[csr.nrows:].any()
+                gotf = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(nbr * T, torch.float32, dev), dtype=torch.float32))
+                np.testing.assert_array_equal(gotf[: csr.nrows], want.astype(np.float32))
+                twice = B.bmv_bin_bin_full(A, xb, out=torch.from_numpy(got).to(dev), accumulate=True)
+                np.testing.assert_array_equal(to_np(twice)[: csr.nrows], 2 * want)
+    finally:
+        bgb.call("bgb_set_row_pieces", prev)
+
+
 @pytest.mark.parametrize("T", TS)
 def test_rowptr_rewritten_in_place(orc, dev, T):
     """The chunk partition is cached per (stream, rowptr address, sizes): a caller
