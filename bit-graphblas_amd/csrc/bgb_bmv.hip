@@ -891,8 +891,8 @@ __global__ __launch_bounds__(256) void bmv_rows_kernel(BmvArgs a) {
 // fixed chunks regardless of rows, and pays for it per chunk: the group-row
 // pass, a row-end test per tile, the segmented scan and carries. Here the
 // work is cut along rows instead, classified once per matrix:
-//   * short block rows (at most kShortRowMax tiles): one lane per row
-//     (block_row_sums), as in the short-row path;
+//   * short block rows (at most kShortRowMax tiles): one lane per row, by the
+//     piece kernel's waves before their pieces (x read as the pieces read it);
 //   * longer rows: pieces of piece_tiles<T>() tiles on a grid aligned to the row's
 //     first 16-byte unit of tile words, one wave per piece: lanes stream
 //     whole 16-byte units (4 tiles at T = 4, 2 at T = 8), count with
@@ -931,35 +931,13 @@ __global__ void piece_fill_kernel(const int *__restrict__ rowptr, const int *__r
   for (int j = 0; j < n; ++j) pieces[o + j] = make_int4((int)I, 2 * j + (j == n - 1 ? 1 : 0), s, e);
 }
 
-// The short rows of a piece plan (np[I] == 0), one lane per block row; empty
-// rows are left to the fix-up's zeroing pass.
-template <int T, int OP>
-__global__ __launch_bounds__(256) void bmv_short_rows_kernel(BmvArgs a, const int *__restrict__ np) {
-  using Acc = typename AccOf<OP>::type;
-  for (long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x; I < a.nbr;
-       I += (long long)gridDim.x * blockDim.x) {
-    const int s = a.rowptr[I], e = a.rowptr[I + 1];
-    if (np[I] != 0 || s == e) continue;
-    Acc acc[Tile<T>::R];
-    block_row_sums<T, kRowCount, 8>(a.A, a.colind, a.B, s, e, acc);
-    flush_row<T, OP>(a, (int)I, 0, acc);
-  }
-}
-
-// Byte-wise popcounts of a word: byte r of the result = bits set in byte r.
-__device__ __forceinline__ uint32_t byte_popc(uint32_t m) {
-  m = m - ((m >> 1) & 0x55555555u);
-  m = (m & 0x33333333u) + ((m >> 2) & 0x33333333u);
-  return (m + (m >> 4)) & 0x0F0F0F0Fu;
-}
-
 // One wave per piece, persistent over the piece list (waves take pieces
 // w, w + nw, ...). XL: packed x in LDS -- 1 all of it, 2 its first
 // a.xl_words words (the rest from memory), 0 none (gathered from memory).
 // A piece's inputs: its tile words and block columns (one 16-byte unit of
 // tile words per lane and step; units outside the piece read nothing, their
-// offsets lying past the buffers), the row's extent now, and the packed-x
-// word of each tile (0 outside the piece).
+// offsets lying past the buffers), the row's extent now, the packed-x read
+// of each tile, and the piece's tile range.
 template <int T>
 struct PieceIn {
   static constexpr int A = T == 4 ? 4 : 2;          // tiles per 16-byte unit
@@ -968,6 +946,7 @@ struct PieceIn {
   uint32_t jv[STEPS][A];
   uint32_t xw[STEPS][A];
   int s_now, e_now;
+  int base, lo, hi;  // the piece's aligned start and tiles [lo, hi) (wave-uniform)
 };
 
 template <int T>
@@ -1012,64 +991,168 @@ __device__ __forceinline__ void piece_load(const BmvArgs &a, __amdgpu_buffer_rsr
 
 // Issues the packed-x reads of piece d's tiles (its block columns loaded):
 // XL 1 all of x staged in LDS, 2 its first a.xl_words words (the others
-// from memory), 0 none.
+// from memory), 0 none. Every slot reads the word of its block column (a
+// slot outside the piece holds a tile of a neighbouring row, or reads zeros
+// past the matrix: either way a valid column); piece_count zeroes the slots
+// outside [lo, hi). With x all in LDS only the loads are issued here (the
+// raw LDS byte is kept; piece_count extracts the word), so their latency
+// overlaps the count of the piece before.
 template <int T, int XL>
 __device__ __forceinline__ void piece_xload(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr, int4 d,
                                             int lane, PieceIn<T> &in) {
   using W = typename Tile<T>::W;
   constexpr int A = PieceIn<T>::A;
   const int base = piece_base<T>(d.z, d.y);
-  const int lo = base > d.z ? base : d.z, hi = piece_hi<T>(d, base);
+  in.base = base;
+  in.lo = base > d.z ? base : d.z;
+  in.hi = piece_hi<T>(d, base);
 #pragma unroll
   for (int st = 0; st < PieceIn<T>::STEPS; ++st)
 #pragma unroll
     for (int u = 0; u < A; ++u) {
-      const int tt = base + (lane + 64 * st) * A + u;
-      const bool inr = tt >= lo && tt < hi;
-      const int J = inr ? (int)in.jv[st][u] : 0;
+      const int J = (int)in.jv[st][u];
       uint32_t x;
       if constexpr (XL == 1) {
-        x = lds_xword<T>(xl, J);
+        x = xl[T == 4 ? (J >> 1) : J];
       } else if constexpr (XL == 2) {
-        const bool staged = !inr || J < a.xl_words;
+        const bool staged = J < a.xl_words;
         const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
         const int off = staged ? a.x_words * (int)sizeof(W) : J * (int)sizeof(W);
         const uint32_t xm = __builtin_amdgcn_raw_buffer_load_b8(xr, off, 0, 0);
         x = staged ? xs : xm;
       } else {
-        x = __builtin_amdgcn_raw_buffer_load_b8(xr, inr ? J * (int)sizeof(W) : a.x_words * (int)sizeof(W), 0, 0);
+        x = __builtin_amdgcn_raw_buffer_load_b8(xr, J * (int)sizeof(W), 0, 0);
       }
-      in.xw[st][u] = inr ? x : 0u;
+      in.xw[st][u] = x;
     }
 }
 
-// Row totals t[] += the counts of a loaded piece (tiles outside it have x 0).
-template <int T>
-__device__ __forceinline__ void piece_count(const PieceIn<T> &in, uint32_t (&t)[T]) {
+// Row totals t[] += the counts of a loaded piece. Per lane: the x word of
+// each of its 4 tiles (0 outside the piece), then the rows' bits of all 4
+// tiles gathered into one register per row with byte permutes, one v_bcnt
+// per row; then one DPP wave sum per two rows (16-bit fields: a piece adds
+// at most 256 * T per row).
+//   T = 4: tile words hold rows 0..3 as the low nibbles of bytes 0..3; tiles
+//          0 and 1 (2 and 3) share a register, tile 1 (3) in the high
+//          nibbles, ANDed with their x words likewise placed; row r's 16 bits
+//          are byte r of both registers.
+//   T = 8: tile k's rows 0..3 and 4..7 are bytes of two registers; a 4x4
+//          byte transpose of the four tiles' low (high) registers gives row
+//          r in one register, byte k from tile k, ANDed with the four x
+//          bytes.
+template <int T, int XL>
+__device__ __forceinline__ void piece_count(const PieceIn<T> &in, int lane, uint32_t (&t)[T]) {
   constexpr int A = PieceIn<T>::A;
-  uint32_t pk[T / 4];
-#pragma unroll
-  for (int q = 0; q < T / 4; ++q) pk[q] = 0;
+  static_assert(PieceIn<T>::STEPS * A == 4, "four tiles per lane");
+  uint32_t xv[4], w[4 * (T / 4)];
 #pragma unroll
   for (int st = 0; st < PieceIn<T>::STEPS; ++st) {
     const uint32_t w4[4] = {in.wv[st].x, in.wv[st].y, in.wv[st].z, in.wv[st].w};
 #pragma unroll
-    for (int u = 0; u < A; ++u) {
-      const uint32_t xr4 = in.xw[st][u] * 0x01010101u;
-      // byte-wise row counts (a lane adds at most piece_tiles / 64 tiles of <= T each)
+    for (int i = 0; i < 4; ++i) w[4 * st + i] = w4[i];
 #pragma unroll
-      for (int q = 0; q < T / 4; ++q) pk[q] += byte_popc(w4[u * (T / 4) + q] & xr4);
+    for (int u = 0; u < A; ++u) {
+      uint32_t x = in.xw[st][u];
+      if constexpr (XL == 1 && T == 4) x = __builtin_amdgcn_ubfe(x, (in.jv[st][u] & 1u) << 2, 4);
+      else if constexpr (T == 4) x &= 0xFu;
+      else x &= 0xFFu;
+      xv[st * A + u] = x;
     }
   }
-  // wave totals, two rows per register in 16-bit fields (at most piece_tiles * T)
+  if (!(in.lo == in.base && in.hi == in.base + piece_tiles<T>())) {  // wave-uniform: slots outside the piece
 #pragma unroll
-  for (int q = 0; q < T / 4; ++q) {
-    const uint32_t ev = wave_total(pk[q] & 0x00FF00FFu);         // rows 4q, 4q + 2
-    const uint32_t od = wave_total((pk[q] >> 8) & 0x00FF00FFu);  // rows 4q + 1, 4q + 3
-    t[4 * q] += ev & 0xFFFFu;
-    t[4 * q + 2] += ev >> 16;
-    t[4 * q + 1] += od & 0xFFFFu;
-    t[4 * q + 3] += od >> 16;
+    for (int st = 0; st < PieceIn<T>::STEPS; ++st)
+#pragma unroll
+      for (int u = 0; u < A; ++u) {
+        const int tt = in.base + (lane + 64 * st) * A + u;
+        xv[st * A + u] = tt >= in.lo && tt < in.hi ? xv[st * A + u] : 0u;
+      }
+  }
+  uint32_t lt[T];
+  if constexpr (T == 4) {
+    const uint32_t c0 = (w[0] & 0x0F0F0F0Fu) | ((w[1] << 4) & 0xF0F0F0F0u);
+    const uint32_t c1 = (w[2] & 0x0F0F0F0Fu) | ((w[3] << 4) & 0xF0F0F0F0u);
+    const uint32_t m0 = c0 & __builtin_amdgcn_perm(0u, xv[0] | (xv[1] << 4), 0u);  // byte 0 to every byte
+    const uint32_t m1 = c1 & __builtin_amdgcn_perm(0u, xv[2] | (xv[3] << 4), 0u);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      lt[r] = __builtin_popcount(__builtin_amdgcn_perm(m1, m0, 0x0C0C0000u | ((4u + r) << 8) | (uint32_t)r));
+  } else {
+    // w: tile 0 rows 0-3, rows 4-7, tile 1 rows 0-3, rows 4-7, tile 2 ..., tile 3 ...
+    const uint32_t X4 = xv[0] | (xv[1] << 8) | (xv[2] << 16) | (xv[3] << 24);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t r0 = w[h], r1 = w[2 + h], r2 = w[4 + h], r3 = w[6 + h];  // tiles 0..3
+      const uint32_t p0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);  // t0.b0 t1.b0 t0.b1 t1.b1
+      const uint32_t p1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);  // t0.b2 t1.b2 t0.b3 t1.b3
+      const uint32_t p2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+      const uint32_t p3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+      lt[4 * h + 0] = __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x05040100u) & X4);
+      lt[4 * h + 1] = __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x07060302u) & X4);
+      lt[4 * h + 2] = __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x05040100u) & X4);
+      lt[4 * h + 3] = __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x07060302u) & X4);
+    }
+  }
+  // wave totals, rows r and r + T/2 in one register
+#pragma unroll
+  for (int r = 0; r < T / 2; ++r) {
+    const uint32_t v = wave_total(lt[r] | (lt[r + T / 2] << 16));
+    t[r] += v & 0xFFFFu;
+    t[r + T / 2] += v >> 16;
+  }
+}
+
+// Packed-x word J as the piece walk reads it: from LDS (XL 1), from LDS when
+// staged and else from memory (XL 2: both loads issued, the memory one with
+// an offset past the buffer when the word is staged), or from memory (XL 0).
+template <int T, int XL>
+__device__ __forceinline__ uint32_t piece_xword(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr,
+                                                int J) {
+  using W = typename Tile<T>::W;
+  if constexpr (XL == 1) {
+    return lds_xword<T>(xl, J);
+  } else if constexpr (XL == 2) {
+    const bool staged = J < a.xl_words;
+    const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
+    const uint32_t xm =
+        __builtin_amdgcn_raw_buffer_load_b8(xr, staged ? a.x_words * (int)sizeof(W) : J * (int)sizeof(W), 0, 0);
+    return staged ? xs : xm;
+  } else {
+    return __builtin_amdgcn_raw_buffer_load_b8(xr, J * (int)sizeof(W), 0, 0);
+  }
+}
+
+// Counts of a short block row (tiles [s, e), s < e) for the piece kernel:
+// batches of 8 tiles, every load of a batch issued unconditionally (indices
+// clamped into the row, the padding slots' words zeroed), x words read as
+// the pieces read them.
+template <int T, int XL>
+__device__ __forceinline__ void short_row_counts(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr,
+                                                 int s, int e, uint32_t (&acc)[T]) {
+  constexpr int U = 8, NW = Tile<T>::NW;
+#pragma unroll
+  for (int i = 0; i < T; ++i) acc[i] = 0;
+  for (int k = s; k < e; k += U) {
+    uint32_t F[U][NW];
+    int J[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = k + u < e ? k + u : e - 1;
+      J[u] = a.colind[p];
+      load_frag<T>(a.A, p, 0, F[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k + u >= e) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) F[u][w] = 0u;
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t x = piece_xword<T, XL>(a, xl, xr, J[u]);
+#pragma unroll
+      for (int i = 0; i < T; ++i) acc[i] += bit_count(row_word<T>(F[u], i) & x);
+    }
   }
 }
 
@@ -1080,7 +1163,7 @@ __device__ __forceinline__ void piece_count(const PieceIn<T> &in, uint32_t (&t)[
 // all of it, 2 its first a.xl_words words (the rest from memory), 0 none.
 template <int T, int OP, int XL>
 __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes, const int4 *__restrict__ pieces,
-                                                         int npieces) {
+                                                         int npieces, const int *__restrict__ np) {
   static_assert(T == 4 || T == 8, "one lane per tile");
   using W = typename Tile<T>::W;
   using Acc = typename AccOf<OP>::type;
@@ -1095,16 +1178,35 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   const int lane = threadIdx.x & 63;
   const int nw = (int)gridDim.x * ((int)blockDim.x >> 6);
   int p = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6));
-  if (p >= npieces) return;  // uniform per wave; no barrier follows
   const int p0 = p;
   const __amdgpu_buffer_rsrc_t cr =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(a.colind), (short)0, a.nblocks * 4, 0x00020000);
   const __amdgpu_buffer_rsrc_t vr =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.A), (short)0, a.nblocks * TB, 0x00020000);
-  __amdgpu_buffer_rsrc_t xr;
-  if constexpr (XL != 1)
-    xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.B), (short)0, a.x_words * (int)sizeof(W),
-                                           0x00020000);
+  const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.B), (short)0,
+                                                                      a.x_words * (int)sizeof(W), 0x00020000);
+  // The short rows (np[I] == 0), one lane each, in blocks of 64 consecutive
+  // block rows, wave w taking blocks w, w + nw, ...: done by the same waves
+  // before their pieces, with x read as the pieces read it (from LDS where
+  // staged), while the wave's first piece loads are in flight. Empty rows
+  // are left to the fix-up's zeroing pass.
+  auto short_rows = [&]() {
+    for (long long blk = p0; blk * 64 < a.nbr; blk += nw) {
+      const long long I = blk * 64 + lane;
+      if (I < a.nbr && np[I] == 0) {
+        const int s = a.rowptr[I], e = a.rowptr[I + 1];
+        if (s < e) {
+          uint32_t acc[T];
+          short_row_counts<T, XL>(a, xl, xr, s, e, acc);
+          flush_row<T, OP>(a, (int)I, 0, acc);
+        }
+      }
+    }
+  };
+  if (p >= npieces) {  // uniform per wave; no barrier follows
+    short_rows();
+    return;
+  }
   // descriptors as vector loads; past the wave's last piece every load
   // (descriptor, tiles, x) repeats its first piece, results unused, so that
   // all loads stay unconditional and the waits count them exactly
@@ -1136,7 +1238,7 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
     for (int r = 0; r < T; ++r) t[r] = 0;
     const int s_now = __builtin_amdgcn_readfirstlane(cur.s_now), e_now = __builtin_amdgcn_readfirstlane(cur.e_now);
     if (s_now == d0.z && e_now == d0.w) {
-      piece_count<T>(cur, t);
+      piece_count<T, XL>(cur, lane, t);
     } else {
       // rowptr was rewritten since the plan: count the row as it is now, a
       // window at a time (the row's last piece runs on to the row's end)
@@ -1148,7 +1250,7 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
         const int4 dw = make_int4(I, 1, kb, min(kb + P, kend));  // one window, from kb
         piece_load<T>(a, vr, cr, dw, lane, w);
         piece_xload<T, XL>(a, xl, xr, make_int4(I, 1, kb > s_now ? kb : s_now, dw.w), lane, w);
-        piece_count<T>(w, t);
+        piece_count<T, XL>(w, lane, t);
       }
       // drain everything here, so that after the join the compiler still
       // counts the prefetches of the common path exactly (rather than
@@ -1175,6 +1277,7 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   PieceIn<T> b0, b1, b2;
   piece_load<T>(a, vr, cr, d0, lane, b0);
   piece_load<T>(a, vr, cr, d1, lane, b1);
+  short_rows();
   piece_xload<T, XL>(a, xl, xr, d0, lane, b0);
   while (step(b0, b1, b2) && step(b1, b2, b0) && step(b2, b0, b1)) {
   }
@@ -1350,9 +1453,10 @@ int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntr
   return BGB_OK;
 }
 
-// The row-piece path's main passes: the short rows (one lane per block row)
-// and the pieces (persistent waves, the packed x staged in LDS when it fits,
-// or its first words when a useful share of it does, else read from memory).
+// The row-piece path's main pass: one persistent launch doing the short rows
+// (one lane per block row) and then the pieces, with the packed x staged in
+// LDS when it fits, or its first words when a useful share of it does, else
+// read from memory.
 template <int T, int OP>
 int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
   using W = typename Tile<T>::W;
@@ -1378,12 +1482,9 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
               x_bytes);
   a.x_words = (int)words;
   a.xl_words = partial ? (int)part_words : 0;
-  bmv_short_rows_kernel<T, OP><<<capped_blocks(a.nbr, 256), 256, 0, s>>>(a, pe.np);
-  BGB_HIP_TRY(hipGetLastError());
-  if (pe.npieces == 0) return BGB_OK;
   const int xlk = fits ? 1 : partial ? 2 : 0;
   const long long lds = xlk == 1 ? xl : xlk == 2 ? lds_max : 0;
-  void (*k)(BmvArgs, int, const int4 *, int) = xlk == 1   ? &bmv_piece_kernel<T, OP, 1>
+  void (*k)(BmvArgs, int, const int4 *, int, const int *) = xlk == 1 ? &bmv_piece_kernel<T, OP, 1>
                                               : xlk == 2 ? &bmv_piece_kernel<T, OP, 2>
                                                          : &bmv_piece_kernel<T, OP, 0>;
   if (lds > 0)
@@ -1392,10 +1493,13 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
   int per_cu = 0;
   BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), 1024,
                                                            (size_t)lds));
-  const unsigned grid = (unsigned)std::min<long long>((long long)f.cus * std::max(per_cu, 1),
-                                                      ceil_div(pe.npieces, 16));
+  // waves: one per 16 pieces or per 16 blocks of 64 short-row candidates,
+  // at most the occupancy
+  const unsigned grid = (unsigned)std::min<long long>(
+      (long long)f.cus * std::max(per_cu, 1),
+      std::max<long long>(1, std::max(ceil_div(pe.npieces, 16), ceil_div(ceil_div(a.nbr, 64), 16))));
   const int stage_bytes = xlk == 1 ? (int)x_bytes : xlk == 2 ? (int)(part_words * (long long)sizeof(W)) : 0;
-  k<<<grid, 1024, (size_t)lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces);
+  k<<<grid, 1024, (size_t)lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces, pe.np);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }

@@ -261,19 +261,19 @@ def _rmat20():
 
 @pytest.mark.parametrize("T", (4, 8))
 def test_row_pieces_modes(orc, dev, T):
-    """The T <= 8 count BMVs cut long block rows into 512-tile pieces aligned to
+    """The T <= 8 count BMVs cut long block rows into 256-tile pieces aligned to
     the row's first 16-byte unit (one wave each, the last piece writing the row,
     the others leaving carry records) and give short rows a lane each. The
     piece path and the chunk walk must both give the oracle's counts: rows of
-    31..34 tiles (the short/long threshold), 511..513 and 1025 tiles (piece
-    edges), row starts at every alignment, an R-MAT graph, accumulate=1."""
+    31..34 tiles (the short/long threshold), 255..257, 511..513 and 1025
+    tiles (piece edges), row starts at every alignment, an R-MAT graph, accumulate=1."""
     import torch
     import bgb
     from bgb import b2sr as B
     from bgb import graph
     rng = np.random.default_rng(70 + T)
     n = 12000 * T // 4 + 3
-    lens = [31, 32, 33, 34, 1, 511, 512, 513, 2, 1025, 3, 700, 0, 5, 64, 65, 40]
+    lens = [31, 32, 33, 34, 1, 255, 256, 257, 511, 512, 513, 2, 1025, 3, 700, 0, 5, 64, 65, 40]
     src, dst = [], []
     for i, k in enumerate(lens):
         cols = rng.choice(n // T, k, replace=False) * T + rng.integers(0, T, k)
