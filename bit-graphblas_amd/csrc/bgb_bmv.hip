@@ -62,6 +62,11 @@ struct BmvArgs {
   int fixup_blocks;  // workgroups of the fix-up pass proper; any after them zero the empty block rows
   int xl_words;      // LDS kernels: packed-x words [0, xl_words) staged, the rest read from B
   int x_words;       // packed-x words in B (the range of its buffer resource)
+  // Piece kernel with a non-zero bitmap (XL 3): bit j of xbm = word xl_words + j
+  // of x is non-zero; bm_dwords dwords (a multiple of 4), staged at LDS byte bm_off.
+  const uint32_t *xbm;
+  int bm_dwords;
+  int bm_off;
 };
 
 constexpr int kWavesPerBlock = 4;
@@ -989,9 +994,38 @@ __device__ __forceinline__ void piece_load(const BmvArgs &a, __amdgpu_buffer_rsr
   in.e_now = a.rowptr[iv + 1];
 }
 
+// Packed-x word J as the piece walk reads it: from LDS (XL 1); from LDS when
+// staged and else from memory (XL 2: both loads issued, the memory one with
+// an offset past the buffer when the word is staged); from memory (XL 0);
+// XL 3: from LDS when staged, else from memory only when the LDS bitmap of
+// the unstaged words says it is non-zero (zero otherwise, with no memory
+// access).
+template <int T, int XL>
+__device__ __forceinline__ uint32_t piece_xword(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr,
+                                                int J) {
+  using W = typename Tile<T>::W;
+  if constexpr (XL == 1) {
+    return lds_xword<T>(xl, J);
+  } else if constexpr (XL == 2 || XL == 3) {
+    const bool staged = J < a.xl_words;
+    const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
+    bool mem = !staged;
+    if constexpr (XL == 3) {
+      const int jb = staged ? 0 : J - a.xl_words;
+      const uint32_t bw = reinterpret_cast<const uint32_t *>(xl + a.bm_off)[jb >> 5];
+      mem = mem && ((bw >> (jb & 31)) & 1u) != 0;
+    }
+    const uint32_t xm =
+        __builtin_amdgcn_raw_buffer_load_b8(xr, mem ? J * (int)sizeof(W) : a.x_words * (int)sizeof(W), 0, 0);
+    return staged ? xs : xm;
+  } else {
+    return __builtin_amdgcn_raw_buffer_load_b8(xr, J * (int)sizeof(W), 0, 0);
+  }
+}
+
 // Issues the packed-x reads of piece d's tiles (its block columns loaded):
 // XL 1 all of x staged in LDS, 2 its first a.xl_words words (the others
-// from memory), 0 none. Every slot reads the word of its block column (a
+// from memory), 3 those and the non-zero bitmap of the others, 0 none. Every slot reads the word of its block column (a
 // slot outside the piece holds a tile of a neighbouring row, or reads zeros
 // past the matrix: either way a valid column); piece_count zeroes the slots
 // outside [lo, hi). With x all in LDS only the loads are issued here (the
@@ -1014,14 +1048,10 @@ __device__ __forceinline__ void piece_xload(const BmvArgs &a, const uint8_t *xl,
       uint32_t x;
       if constexpr (XL == 1) {
         x = xl[T == 4 ? (J >> 1) : J];
-      } else if constexpr (XL == 2) {
-        const bool staged = J < a.xl_words;
-        const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
-        const int off = staged ? a.x_words * (int)sizeof(W) : J * (int)sizeof(W);
-        const uint32_t xm = __builtin_amdgcn_raw_buffer_load_b8(xr, off, 0, 0);
-        x = staged ? xs : xm;
-      } else {
+      } else if constexpr (XL == 0) {
         x = __builtin_amdgcn_raw_buffer_load_b8(xr, J * (int)sizeof(W), 0, 0);
+      } else {
+        x = piece_xword<T, XL>(a, xl, xr, J);
       }
       in.xw[st][u] = x;
     }
@@ -1102,26 +1132,6 @@ __device__ __forceinline__ void piece_count(const PieceIn<T> &in, int lane, uint
   }
 }
 
-// Packed-x word J as the piece walk reads it: from LDS (XL 1), from LDS when
-// staged and else from memory (XL 2: both loads issued, the memory one with
-// an offset past the buffer when the word is staged), or from memory (XL 0).
-template <int T, int XL>
-__device__ __forceinline__ uint32_t piece_xword(const BmvArgs &a, const uint8_t *xl, __amdgpu_buffer_rsrc_t xr,
-                                                int J) {
-  using W = typename Tile<T>::W;
-  if constexpr (XL == 1) {
-    return lds_xword<T>(xl, J);
-  } else if constexpr (XL == 2) {
-    const bool staged = J < a.xl_words;
-    const uint32_t xs = lds_xword<T>(xl, staged ? J : 0);
-    const uint32_t xm =
-        __builtin_amdgcn_raw_buffer_load_b8(xr, staged ? a.x_words * (int)sizeof(W) : J * (int)sizeof(W), 0, 0);
-    return staged ? xs : xm;
-  } else {
-    return __builtin_amdgcn_raw_buffer_load_b8(xr, J * (int)sizeof(W), 0, 0);
-  }
-}
-
 // Counts of a short block row (tiles [s, e), s < e) for the piece kernel:
 // batches of 8 tiles, every load of a batch issued unconditionally (indices
 // clamped into the row, the padding slots' words zeroed), x words read as
@@ -1160,7 +1170,9 @@ __device__ __forceinline__ void short_row_counts(const BmvArgs &a, const uint8_t
 // w, w + nw, ...), software-pipelined three deep: while piece p is counted,
 // the x words of the wave's next piece and the tile words, block columns
 // and row extent of the one after are in flight. XL: packed x in LDS -- 1
-// all of it, 2 its first a.xl_words words (the rest from memory), 0 none.
+// all of it, 2 its first a.xl_words words (the rest from memory), 3 those
+// and the non-zero bitmap of the rest (only non-zero words read from
+// memory), 0 none.
 template <int T, int OP, int XL>
 __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes, const int4 *__restrict__ pieces,
                                                          int npieces, const int *__restrict__ np) {
@@ -1173,6 +1185,11 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   uint8_t *xl = s_dyn;
   if constexpr (XL != 0) {
     stage_x<T>(a, x_bytes, xl);
+    if constexpr (XL == 3) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(a.xbm);
+      uint4 *dst = reinterpret_cast<uint4 *>(xl + a.bm_off);
+      for (int i = threadIdx.x; i < a.bm_dwords / 4; i += blockDim.x) dst[i] = src[i];
+    }
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
@@ -1453,12 +1470,37 @@ int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntr
   return BGB_OK;
 }
 
-// The row-piece path's main pass: one persistent launch doing the short rows
-// (one lane per block row) and then the pieces, with the packed x staged in
-// LDS when it fits, or its first words when a useful share of it does, else
-// read from memory.
-template <int T, int OP>
-int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
+// Non-zero bitmap of the packed-x words past the first `from`: bit j of
+// bm[j / 32] = (word from + j != 0) (T = 4: its low nibble), bm_dwords
+// dwords (those past the words written as 0). One wave per 64 words, a
+// ballot each.
+template <int T>
+__global__ __launch_bounds__(256) void x_nonzero_bitmap_kernel(const uint8_t *__restrict__ x, int from, int words,
+                                                               uint32_t *__restrict__ bm, int bm_dwords) {
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (2 * w >= bm_dwords) return;  // uniform per wave
+  const long long j = from + w * 64 + lane;
+  uint32_t v = j < words ? x[j] : 0u;
+  if constexpr (T == 4) v &= 0xFu;
+  const unsigned long long m = __ballot(v != 0u);
+  if (lane < 2 && 2 * w + lane < bm_dwords) bm[2 * w + lane] = (uint32_t)(m >> (32 * lane));
+}
+
+// Where the piece kernel reads the packed x from (XL of bmv_piece_kernel):
+// 1 all of it in LDS; 3 its first `words` words in LDS with a bitmap of the
+// others' non-zero words (read from memory only where non-zero), when that
+// fits and x is larger than LDS; 2 its first words in LDS, the rest from
+// memory; 0 all from memory. lds: dynamic LDS bytes; bm_bytes: global
+// bitmap bytes the launch needs (XL 3).
+struct PieceX {
+  int xl = 0;
+  long long words = 0, x_bytes = 0, stage_words = 0, lds = 0;
+  int bm_dwords = 0, bm_off = 0;
+  size_t bm_bytes = 0;
+};
+template <int T>
+int piece_x_plan(const BmvArgs &a, hipStream_t s, PieceX *px) {
   using W = typename Tile<T>::W;
   DeviceFacts f;
   int st = device_facts(&f);
@@ -1471,35 +1513,84 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe) {
   BGB_REQUIRE(maxcol >= 0 || a.nblocks == 0, BGB_ERR_ARG, "largest block column unknown (stream capture?)");
   const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
   const long long x_bytes = words * (long long)sizeof(W);
-  const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;  // LDS bytes of all of x
-  const long long lds_max = f.lds_per_block & ~15ll;
-  const long long part_words = T == 4 ? 2 * lds_max : lds_max / (long long)sizeof(W);
-  const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
-  const bool fits = mode != 1 && aligned && xl <= lds_max;
-  const bool partial = mode != 1 && aligned && !fits && x_bytes < (1ll << 31) &&
-                       (mode == 2 || 8 * part_words >= words);
   BGB_REQUIRE(x_bytes < (1ll << 31), BGB_ERR_SIZE, "packed x of %lld bytes past a buffer resource's range",
               x_bytes);
-  a.x_words = (int)words;
-  a.xl_words = partial ? (int)part_words : 0;
-  const int xlk = fits ? 1 : partial ? 2 : 0;
-  const long long lds = xlk == 1 ? xl : xlk == 2 ? lds_max : 0;
-  void (*k)(BmvArgs, int, const int4 *, int, const int *) = xlk == 1 ? &bmv_piece_kernel<T, OP, 1>
-                                              : xlk == 2 ? &bmv_piece_kernel<T, OP, 2>
-                                                         : &bmv_piece_kernel<T, OP, 0>;
-  if (lds > 0)
+  const long long xl = ((T == 4 ? (x_bytes + 1) / 2 : x_bytes) + 15) & ~15ll;  // LDS bytes of all of x
+  const long long lds_max = f.lds_per_block & ~15ll;
+  const bool aligned = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0;
+  PieceX p;
+  p.words = words;
+  p.x_bytes = x_bytes;
+  if (mode != 1 && aligned && xl <= lds_max) {
+    p.xl = 1;
+    p.stage_words = words;
+    p.lds = xl;
+  } else if (mode != 1 && aligned) {
+    // bitmap: prefix S words (S/2 LDS bytes at T = 4) + (words - S)/8 bytes
+    // of bitmap within lds_max; S a multiple of 128 (16-byte prefix)
+    const long long num = 8 * lds_max - words;
+    long long S = num <= 0 ? -1 : ((T == 4 ? num / 3 : num / 7) & ~127ll);
+    auto pre_bytes = [&](long long sw) { return ((T == 4 ? (sw + 1) / 2 : sw) + 15) & ~15ll; };
+    auto bm_dw = [&](long long sw) { return (ceil_div(words - sw, 32) + 3) & ~3ll; };
+    while (S >= 0 && pre_bytes(S) + 4 * bm_dw(S) > lds_max) S -= 128;  // the bitmap's padding
+    if (S >= 0) {
+      p.xl = 3;
+      p.stage_words = S;
+      p.bm_dwords = (int)bm_dw(S);
+      p.bm_off = (int)pre_bytes(S);
+      p.lds = p.bm_off + 4ll * p.bm_dwords;
+      p.bm_bytes = (size_t)(4ll * p.bm_dwords);
+    }
+    if (p.xl == 0) {
+      const long long part_words = T == 4 ? 2 * lds_max : lds_max / (long long)sizeof(W);
+      if (mode == 2 || 8 * part_words >= words) {
+        p.xl = 2;
+        p.stage_words = part_words;
+        p.lds = lds_max;
+      }
+    }
+  }
+  if (p.xl == 0) p.lds = 0;
+  *px = p;
+  return BGB_OK;
+}
+
+// The row-piece path's main pass: one persistent launch doing the short rows
+// (one lane per block row) and then the pieces, the packed x placed as
+// piece_x_plan decided (XL 3: the bitmap made first, into `bm`).
+template <int T, int OP>
+int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe, const PieceX &px, uint32_t *bm) {
+  using W = typename Tile<T>::W;
+  DeviceFacts f;
+  int st = device_facts(&f);
+  if (st) return st;
+  a.x_words = (int)px.words;
+  a.xl_words = px.xl == 2 || px.xl == 3 ? (int)px.stage_words : 0;
+  a.xbm = bm;
+  a.bm_dwords = px.bm_dwords;
+  a.bm_off = px.bm_off;
+  if (px.xl == 3) {
+    x_nonzero_bitmap_kernel<T><<<(unsigned)ceil_div((long long)px.bm_dwords * 32, 256), 256, 0, s>>>(
+        static_cast<const uint8_t *>(a.B), a.xl_words, a.x_words, bm, px.bm_dwords);
+    BGB_HIP_TRY(hipGetLastError());
+  }
+  void (*k)(BmvArgs, int, const int4 *, int, const int *) = px.xl == 1   ? &bmv_piece_kernel<T, OP, 1>
+                                                          : px.xl == 2 ? &bmv_piece_kernel<T, OP, 2>
+                                                          : px.xl == 3 ? &bmv_piece_kernel<T, OP, 3>
+                                                                       : &bmv_piece_kernel<T, OP, 0>;
+  if (px.lds > 0)
     BGB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
+                                    (int)px.lds));
   int per_cu = 0;
   BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), 1024,
-                                                           (size_t)lds));
+                                                           (size_t)px.lds));
   // waves: one per 16 pieces or per 16 blocks of 64 short-row candidates,
   // at most the occupancy
   const unsigned grid = (unsigned)std::min<long long>(
       (long long)f.cus * std::max(per_cu, 1),
       std::max<long long>(1, std::max(ceil_div(pe.npieces, 16), ceil_div(ceil_div(a.nbr, 64), 16))));
-  const int stage_bytes = xlk == 1 ? (int)x_bytes : xlk == 2 ? (int)(part_words * (long long)sizeof(W)) : 0;
-  k<<<grid, 1024, (size_t)lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces, pe.np);
+  const int stage_bytes = px.xl == 0 ? 0 : (int)(px.stage_words * (long long)sizeof(W));
+  k<<<grid, 1024, (size_t)px.lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces, pe.np);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }
@@ -1516,28 +1607,22 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   // its plan is cached and made with a read-back).
   constexpr bool kPieceOp = (T == 4 || T == 8) && (OP == OP_COUNT_I32 || OP == OP_COUNT_F32);
   PieceEntry pe{};
+  PieceX px;
   bool pieces = false;
   if constexpr (kPieceOp) {
     if (!scr && g_row_pieces != 1 && (long long)a.nblocks * 8 < (1ll << 31)) {
-      // automatic: when the whole packed x fits in LDS (with x partly in
-      // memory the pieces' per-slot x reads cost more than the walk saves:
-      // R-MAT 22, T = 4, 405 vs 387 us) and the pieces are mostly full
-      bool want = g_row_pieces == 2;
-      if (!want && g_x_staging[ilog2_tiledim(T) - 2] != 1) {
-        int maxcol = -1;
-        DeviceFacts f;
-        if ((st = device_facts(&f)) || (st = max_block_col(s, a.colind, a.nblocks, &maxcol))) return st;
-        using W = typename Tile<T>::W;
-        const long long words = std::max<long long>(ceil_div(a.nbr, T) * T, (long long)maxcol + 1);
-        const long long xb = words * (long long)sizeof(W);
-        const long long xl = ((T == 4 ? (xb + 1) / 2 : xb) + 15) & ~15ll;
-        want = maxcol >= 0 && (reinterpret_cast<uintptr_t>(a.B) & 15) == 0 && xl <= (f.lds_per_block & ~15ll);
-      }
+      // automatic: when the whole packed x fits in LDS, or its first words and
+      // the non-zero bitmap of the others do (XL 3; with x partly in memory
+      // and no bitmap the pieces' per-slot x reads cost more than the walk
+      // saves: R-MAT 22, T = 4, 396 vs 385 us), and the pieces are well
+      // filled (R-MAT 20, T = 8: 71 % full, pieces 74 us, chunk walk 83 us)
+      if ((st = piece_x_plan<T>(a, s, &px))) return st;
+      bool want = g_row_pieces == 2 || (g_x_staging[ilog2_tiledim(T) - 2] != 1 && (px.xl == 1 || px.xl == 3));
       if (want) {
         if ((st = piece_plan<T>(s, a.rowptr, a.nbr, a.nblocks, &pe))) return st;
         // tiles per piece slot (short rows' tiles included: a bound)
         const double fill = (double)a.nblocks / ((double)std::max(pe.npieces, 1) * piece_tiles<T>());
-        pieces = g_row_pieces == 2 || fill >= 0.75;
+        pieces = g_row_pieces == 2 || fill >= 0.6;
       }
     }
   }
@@ -1548,11 +1633,14 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   const size_t key_bytes = (sizeof(int) * (size_t)std::max(nchunks, 1ll) + 255) & ~(size_t)255;
   const size_t val_bytes = (sizeof(Acc) * (size_t)std::max(nchunks, 1ll) * T + 255) & ~(size_t)255;
   if (pieces) {
-    if ((st = stream_workspace(s, key_bytes + val_bytes, &ws))) return st;
+    const size_t bm_bytes = (px.bm_bytes + 255) & ~(size_t)255;
+    if ((st = stream_workspace(s, key_bytes + val_bytes + bm_bytes, &ws))) return st;
     a.carry_key = static_cast<int *>(ws);
     a.carry_val = static_cast<char *>(ws) + key_bytes;
+    uint32_t *bm = px.bm_bytes ? reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + key_bytes + val_bytes)
+                               : nullptr;
     if constexpr (kPieceOp)
-      if ((st = launch_pieces<T, OP>(a, s, pe))) return st;
+      if ((st = launch_pieces<T, OP>(a, s, pe, px, bm))) return st;
   } else if (scr) {
     const size_t need = 2 * key_bytes + val_bytes;  // keys, partition, records
     BGB_REQUIRE(need <= kScratchBytes, BGB_ERR_SIZE,

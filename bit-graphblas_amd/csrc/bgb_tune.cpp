@@ -134,8 +134,8 @@ const PathFit kFpChunk = {12.0, {6.0, 7.7, 27.0, 52.0}};
 // LDS (R-MAT 22: about 55 % of the tiles' block columns staged).
 const PathFit kBitPart = {15.0, {3.25, 3.7, 8.3, 0}};
 // bin_bin_full at T <= 8 with the whole packed x in LDS: the row-piece path
-// (R-MAT 20: T = 4 68.8 us, T = 8 80.3 us).
-const PathFit kBitPiece = {8.0, {2.24, 3.05, 0, 0}};
+// (R-MAT 20: T = 4 59.4 us, T = 8 74.2 us).
+const PathFit kBitPiece = {8.0, {1.89, 2.79, 0, 0}};
 
 // Packed-x placement of the bit op: 0 gathered from memory, 1 all of it in
 // LDS, 2 its first words in LDS (the launch rule of the BMV kernels: T <= 16

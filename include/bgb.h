@@ -137,9 +137,11 @@ int bgb_get_run_length(int T);
 /* Where the bit-vector BMVs (bin_bin_bin, bin_bin_full, masked) read the
  * packed x for tile dim T: 0 = automatic (the whole packed x is staged in LDS
  * by one persistent workgroup per CU when it fits -- about 140 KB of packed
- * x, i.e. n up to ~1.1M -- and the tile stream is large next to it), 1 =
- * always gathered from memory, 2 = staged whenever it fits. Process-wide
- * tuning knob (no reference counterpart). Default 0. */
+ * x, i.e. n up to ~1.1M -- and the tile stream is large next to it; a larger
+ * x has its first words staged, and for the bin_bin_full row pieces also a
+ * bitmap of which of the other words are non-zero, so that only those are
+ * read from memory), 1 = always gathered from memory, 2 = staged whenever it
+ * fits. Process-wide tuning knob (no reference counterpart). Default 0. */
 int bgb_set_x_staging(int T, int mode);
 int bgb_get_x_staging(int T);
 
@@ -163,8 +165,9 @@ int bgb_get_short_row_path(void);
  * made on the first call for a (rowptr, nblockrows, nblocks, T) -- a count,
  * a scan and one stream synchronisation -- and kept; the kernels read the
  * row extents from rowptr itself, so a rowptr rewritten in place still gives
- * exact results. 0 = automatic: taken when the whole packed x fits in LDS
- * and the pieces are mostly full (otherwise the chunk walk); 1 = never,
+ * exact results. 0 = automatic: taken when the whole packed x, or its first
+ * words and the non-zero bitmap of the others, fit in LDS and the pieces are
+ * well filled (otherwise the chunk walk); 1 = never,
  * 2 = always. Process-wide tuning knob (no reference counterpart). Default 0.
  * Same results on either path. */
 int bgb_set_row_pieces(int mode);
