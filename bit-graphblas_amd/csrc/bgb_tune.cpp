@@ -136,6 +136,12 @@ const PathFit kBitPart = {15.0, {3.25, 3.7, 8.3, 0}};
 // bin_bin_full at T <= 8 with the whole packed x in LDS: the row-piece path
 // (R-MAT 20: T = 4 59.4 us, T = 8 74.2 us).
 const PathFit kBitPiece = {8.0, {1.89, 2.79, 0, 0}};
+// The row-piece path with x larger than LDS: its first words and a bitmap of
+// the others' non-zero words in LDS (R-MAT 22: T = 4 334 us, T = 8 381 us).
+const PathFit kBitPieceBm = {8.0, {2.82, 3.56, 0, 0}};
+// Packed-x words whose bitmap (one bit each, with a staged prefix) fits in
+// the piece kernel's 160 KB of LDS.
+constexpr long long kBitmapWords = 8ll * 160 * 1024;
 
 // Packed-x placement of the bit op: 0 gathered from memory, 1 all of it in
 // LDS, 2 its first words in LDS (the launch rule of the BMV kernels: T <= 16
@@ -164,7 +170,13 @@ double predict_us(int op, int t, int S, const bgb_tile_profile &p, int *x_stagin
   if (op == 0 || op == 1) {
     const PathFit *f;
     if (short_rows) f = op == 0 ? &kBitShort : &kFpShort;
-    else if (op == 0 && lds_path(T, p)) {
+    else if (op == 0 && T <= 8 && !uniform && lds_x_bytes(T, p.nrows) > kLdsForX &&
+             (nbr + T - 1) / T * T < kBitmapWords) {
+      // (skewed rows: long ones fill their pieces; uniform rows of a few
+      // dozen tiles leave pieces mostly empty and keep the chunk walk)
+      f = &kBitPieceBm;
+      if (x_staging) *x_staging = 2;
+    } else if (op == 0 && lds_path(T, p)) {
       f = lds_path(T, p) == 1 ? (T <= 8 ? &kBitPiece : &kBitLds) : &kBitPart;
       if (x_staging) *x_staging = 2;
     } else f = op == 0 ? &kBitMem : &kFpChunk;
