@@ -1206,17 +1206,19 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   // block rows, wave w taking blocks w, w + nw, ...: done by the same waves
   // before their pieces, with x read as the pieces read it (from LDS where
   // staged), while the wave's first piece loads are in flight. Empty rows
-  // are left to the fix-up's zeroing pass.
+  // get zeros when C is overwritten (the fix-up zeroes none on this path).
   auto short_rows = [&]() {
     for (long long blk = p0; blk * 64 < a.nbr; blk += nw) {
       const long long I = blk * 64 + lane;
       if (I < a.nbr && np[I] == 0) {
         const int s = a.rowptr[I], e = a.rowptr[I + 1];
-        if (s < e) {
-          uint32_t acc[T];
-          short_row_counts<T, XL>(a, xl, xr, s, e, acc);
-          flush_row<T, OP>(a, (int)I, 0, acc);
+        uint32_t acc[T];
+        if (s < e) short_row_counts<T, XL>(a, xl, xr, s, e, acc);
+        else {
+#pragma unroll
+          for (int r = 0; r < T; ++r) acc[r] = 0;
         }
+        if (s < e || !a.accumulate) flush_row<T, OP>(a, (int)I, 0, acc);
       }
     }
   };
@@ -1731,8 +1733,11 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
   constexpr bool kBits = OP == OP_BITS || OP == OP_MASKED;
   const long long fix_blocks = kBits ? ceil_div(nchunks, 256) + 1 : ceil_div(ceil_div(nchunks, 64) * 64, 256);
   a.fixup_blocks = (int)fix_blocks;
-  // overwriting calls also zero the empty block rows (accumulate leaves them)
-  const long long zero_blocks = (a.accumulate && OP != OP_MASKED) ? 0 : std::min<long long>(ceil_div(a.nbr, 256), 1024);
+  // overwriting calls also zero the empty block rows (accumulate leaves
+  // them); the piece kernel zeroes its own
+  const long long zero_blocks = ((a.accumulate && OP != OP_MASKED) || pieces)
+                                    ? 0
+                                    : std::min<long long>(ceil_div(a.nbr, 256), 1024);
   if (fix_blocks + zero_blocks == 0) return BGB_OK;  // no records (all rows short) and nothing to zero
   bmv_fixup_kernel<T, OP><<<(unsigned)(fix_blocks + zero_blocks), 256, 0, s>>>(a);
   BGB_HIP_TRY(hipGetLastError());
