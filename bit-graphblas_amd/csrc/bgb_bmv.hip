@@ -1072,58 +1072,61 @@ __device__ __forceinline__ void piece_xload(const BmvArgs &a, const uint8_t *xl,
 //          bytes.
 template <int T, int XL>
 __device__ __forceinline__ void piece_count(const PieceIn<T> &in, int lane, uint32_t (&t)[T]) {
-  constexpr int A = PieceIn<T>::A;
-  static_assert(PieceIn<T>::STEPS * A == 4, "four tiles per lane");
-  uint32_t xv[4], w[4 * (T / 4)];
+  constexpr int A = PieceIn<T>::A, STEPS = PieceIn<T>::STEPS;
+  constexpr int G = STEPS * A / 4;  // groups of 4 tiles per lane
+  static_assert(STEPS * A % 4 == 0, "whole groups of four tiles per lane");
+  const bool edge = !(in.lo == in.base && in.hi == in.base + piece_tiles<T>());  // wave-uniform
+  uint32_t lt[T];
 #pragma unroll
-  for (int st = 0; st < PieceIn<T>::STEPS; ++st) {
-    const uint32_t w4[4] = {in.wv[st].x, in.wv[st].y, in.wv[st].z, in.wv[st].w};
+  for (int r = 0; r < T; ++r) lt[r] = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w[4 * st + i] = w4[i];
+  for (int g = 0; g < G; ++g) {
+    uint32_t xv[4], w[4 * (T / 4)];
 #pragma unroll
-    for (int u = 0; u < A; ++u) {
+    for (int k = 0; k < 4; ++k) {
+      const int st = (4 * g + k) / A, u = (4 * g + k) % A;
       uint32_t x = in.xw[st][u];
       if constexpr (XL == 1 && T == 4) x = __builtin_amdgcn_ubfe(x, (in.jv[st][u] & 1u) << 2, 4);
       else if constexpr (T == 4) x &= 0xFu;
       else x &= 0xFFu;
-      xv[st * A + u] = x;
-    }
-  }
-  if (!(in.lo == in.base && in.hi == in.base + piece_tiles<T>())) {  // wave-uniform: slots outside the piece
-#pragma unroll
-    for (int st = 0; st < PieceIn<T>::STEPS; ++st)
-#pragma unroll
-      for (int u = 0; u < A; ++u) {
+      if (edge) {  // slots outside the piece count nothing
         const int tt = in.base + (lane + 64 * st) * A + u;
-        xv[st * A + u] = tt >= in.lo && tt < in.hi ? xv[st * A + u] : 0u;
+        x = tt >= in.lo && tt < in.hi ? x : 0u;
       }
-  }
-  uint32_t lt[T];
-  if constexpr (T == 4) {
-    const uint32_t c0 = (w[0] & 0x0F0F0F0Fu) | ((w[1] << 4) & 0xF0F0F0F0u);
-    const uint32_t c1 = (w[2] & 0x0F0F0F0Fu) | ((w[3] << 4) & 0xF0F0F0F0u);
-    const uint32_t m0 = c0 & __builtin_amdgcn_perm(0u, xv[0] | (xv[1] << 4), 0u);  // byte 0 to every byte
-    const uint32_t m1 = c1 & __builtin_amdgcn_perm(0u, xv[2] | (xv[3] << 4), 0u);
+      xv[k] = x;
+    }
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      lt[r] = __builtin_popcount(__builtin_amdgcn_perm(m1, m0, 0x0C0C0000u | ((4u + r) << 8) | (uint32_t)r));
-  } else {
-    // w: tile 0 rows 0-3, rows 4-7, tile 1 rows 0-3, rows 4-7, tile 2 ..., tile 3 ...
-    const uint32_t X4 = xv[0] | (xv[1] << 8) | (xv[2] << 16) | (xv[3] << 24);
+    for (int q = 0; q < T / 4; ++q) {
+      const uint4 v = in.wv[g * (T / 4) + q];
+      w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
+    }
+    if constexpr (T == 4) {
+      const uint32_t c0 = (w[0] & 0x0F0F0F0Fu) | ((w[1] << 4) & 0xF0F0F0F0u);
+      const uint32_t c1 = (w[2] & 0x0F0F0F0Fu) | ((w[3] << 4) & 0xF0F0F0F0u);
+      const uint32_t m0 = c0 & __builtin_amdgcn_perm(0u, xv[0] | (xv[1] << 4), 0u);  // byte 0 to every byte
+      const uint32_t m1 = c1 & __builtin_amdgcn_perm(0u, xv[2] | (xv[3] << 4), 0u);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t r0 = w[h], r1 = w[2 + h], r2 = w[4 + h], r3 = w[6 + h];  // tiles 0..3
-      const uint32_t p0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);  // t0.b0 t1.b0 t0.b1 t1.b1
-      const uint32_t p1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);  // t0.b2 t1.b2 t0.b3 t1.b3
-      const uint32_t p2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
-      const uint32_t p3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
-      lt[4 * h + 0] = __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x05040100u) & X4);
-      lt[4 * h + 1] = __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x07060302u) & X4);
-      lt[4 * h + 2] = __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x05040100u) & X4);
-      lt[4 * h + 3] = __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x07060302u) & X4);
+      for (int r = 0; r < 4; ++r)
+        lt[r] += __builtin_popcount(__builtin_amdgcn_perm(m1, m0, 0x0C0C0000u | ((4u + r) << 8) | (uint32_t)r));
+    } else {
+      // w: tile 0 rows 0-3, rows 4-7, tile 1 rows 0-3, rows 4-7, tile 2 ..., tile 3 ...
+      const uint32_t X4 = xv[0] | (xv[1] << 8) | (xv[2] << 16) | (xv[3] << 24);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t r0 = w[h], r1 = w[2 + h], r2 = w[4 + h], r3 = w[6 + h];  // tiles 0..3
+        const uint32_t p0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);  // t0.b0 t1.b0 t0.b1 t1.b1
+        const uint32_t p1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);  // t0.b2 t1.b2 t0.b3 t1.b3
+        const uint32_t p2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+        const uint32_t p3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+        lt[4 * h + 0] += __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x05040100u) & X4);
+        lt[4 * h + 1] += __builtin_popcount(__builtin_amdgcn_perm(p2, p0, 0x07060302u) & X4);
+        lt[4 * h + 2] += __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x05040100u) & X4);
+        lt[4 * h + 3] += __builtin_popcount(__builtin_amdgcn_perm(p3, p1, 0x07060302u) & X4);
+      }
     }
   }
-  // wave totals, rows r and r + T/2 in one register
+  // wave totals, rows r and r + T/2 in one register (a lane adds at most
+  // piece_tiles / 64 * T per row, a wave piece_tiles * T)
 #pragma unroll
   for (int r = 0; r < T / 2; ++r) {
     const uint32_t v = wave_total(lt[r] | (lt[r + T / 2] << 16));
@@ -1474,19 +1477,30 @@ int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntr
 
 // Non-zero bitmap of the packed-x words past the first `from`: bit j of
 // bm[j / 32] = (word from + j != 0) (T = 4: its low nibble), bm_dwords
-// dwords (those past the words written as 0). One wave per 64 words, a
-// ballot each.
+// dwords (those past the words written as 0). One thread per 16 words (one
+// 16-byte load; x and `from` are 16-byte aligned), two threads per dword.
 template <int T>
 __global__ __launch_bounds__(256) void x_nonzero_bitmap_kernel(const uint8_t *__restrict__ x, int from, int words,
                                                                uint32_t *__restrict__ bm, int bm_dwords) {
-  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (2 * w >= bm_dwords) return;  // uniform per wave
-  const long long j = from + w * 64 + lane;
-  uint32_t v = j < words ? x[j] : 0u;
-  if constexpr (T == 4) v &= 0xFu;
-  const unsigned long long m = __ballot(v != 0u);
-  if (lane < 2 && 2 * w + lane < bm_dwords) bm[2 * w + lane] = (uint32_t)(m >> (32 * lane));
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long j0 = from + 16 * t;
+  uint32_t m = 0;  // bit k: word j0 + k non-zero
+  if (j0 + 16 <= words) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(x + j0);
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t b = (w4[k >> 2] >> (8 * (k & 3))) & (T == 4 ? 0xFu : 0xFFu);
+      m |= (b != 0u ? 1u : 0u) << k;
+    }
+  } else {
+    for (int k = 0; k < 16 && j0 + k < words; ++k) {
+      const uint32_t b = x[j0 + k] & (T == 4 ? 0xFu : 0xFFu);
+      m |= (b != 0u ? 1u : 0u) << k;
+    }
+  }
+  const uint32_t hi = __shfl_xor(m, 1);  // the odd thread's 16 words
+  if ((t & 1) == 0 && t / 2 < bm_dwords) bm[t / 2] = m | (hi << 16);
 }
 
 // Where the piece kernel reads the packed x from (XL of bmv_piece_kernel):
@@ -1572,7 +1586,7 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe, const PieceX &
   a.bm_dwords = px.bm_dwords;
   a.bm_off = px.bm_off;
   if (px.xl == 3) {
-    x_nonzero_bitmap_kernel<T><<<(unsigned)ceil_div((long long)px.bm_dwords * 32, 256), 256, 0, s>>>(
+    x_nonzero_bitmap_kernel<T><<<(unsigned)ceil_div((long long)px.bm_dwords * 2, 256), 256, 0, s>>>(
         static_cast<const uint8_t *>(a.B), a.xl_words, a.x_words, bm, px.bm_dwords);
     BGB_HIP_TRY(hipGetLastError());
   }

@@ -305,6 +305,52 @@ def test_row_pieces_modes(orc, dev, T):
         bgb.call("bgb_set_row_pieces", prev)
 
 
+@pytest.mark.parametrize("T", (4, 8))
+def test_row_pieces_bitmap_x(orc, dev, T):
+    """A packed x larger than LDS on the row-piece path: LDS holds the first
+    words of x and a bitmap of which of the others are non-zero; only those
+    are read from memory. 3 M vertices (x: 375 KB at T = 4 and 8), columns
+    skewed to low ids plus uniform ones, hub rows of up to 40 000 tiles, a
+    square and a wide matrix; x of density 0.05, all ones (every unstaged
+    word read) and all zeros (none read); overwrite on a dirty output and
+    accumulate; pieces forced and automatic, against the oracle."""
+    import torch
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(110 + T)
+    n = 3000000
+    m = 4000000
+    src = rng.integers(0, n, m)
+    dst = np.where(rng.random(m) < 0.6, rng.integers(0, n // 6, m), rng.integers(0, n, m))
+    hubs = np.repeat(np.array([5, n // 2 + 1, n - 2]), 60000)
+    far = rng.integers(0, n, hubs.size)
+    src = np.concatenate([src, hubs]).astype(np.int32)
+    dst = np.concatenate([dst, far]).astype(np.int32)
+    cases = (("square", graph.from_coo(n, n, src, dst, False, True)),
+             ("wide 1000000 x 3000000", graph.from_coo(1000000, n, src % 1000000, dst, False, True)))
+    prev = bgb.lib().bgb_get_row_pieces()
+    try:
+        for mode in (2, 0):
+            bgb.call("bgb_set_row_pieces", mode)
+            for label, csr in cases:
+                A = B.B2SR.from_csr(csr, T, dev)
+                nbr = A.nblockrows
+                for xlabel, x01 in (("density 0.05", (rng.random(csr.ncols) < 0.05).astype(np.float32)),
+                                    ("all ones", np.ones(csr.ncols, np.float32)),
+                                    ("all zeros", np.zeros(csr.ncols, np.float32))):
+                    log(f"[case] bitmap x, pieces mode {mode}, {label}, x {xlabel}, T={T}")
+                    xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+                    want = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+                    got = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(nbr * T, torch.int32, dev)))
+                    np.testing.assert_array_equal(got[: csr.nrows], want)
+                    assert not got[csr.nrows:].any()
+                    twice = B.bmv_bin_bin_full(A, xb, out=torch.from_numpy(got).to(dev), accumulate=True)
+                    np.testing.assert_array_equal(to_np(twice)[: csr.nrows], 2 * want)
+    finally:
+        bgb.call("bgb_set_row_pieces", prev)
+
+
 @pytest.mark.parametrize("T", TS)
 def test_rowptr_rewritten_in_place(orc, dev, T):
     """The chunk partition is cached per (stream, rowptr address, sizes): a caller
