@@ -1170,9 +1170,12 @@ __device__ __forceinline__ void short_row_counts(const BmvArgs &a, const uint8_t
 }
 
 // One wave per piece, persistent over the piece list (waves take pieces
-// w, w + nw, ...), software-pipelined three deep: while piece p is counted,
+// w, w + nw, ...), software-pipelined four deep: while piece p is counted,
 // the x words of the wave's next piece and the tile words, block columns
-// and row extent of the one after are in flight. XL: packed x in LDS -- 1
+// and row extents of the two after it are in flight. Each step issues the
+// descriptor load first and the tile loads last, so that waiting for a
+// descriptor or for x never waits for the tiles issued after it (the load
+// counter counts in issue order). XL: packed x in LDS -- 1
 // all of it, 2 its first a.xl_words words (the rest from memory), 3 those
 // and the non-zero bitmap of the rest (only non-zero words read from
 // memory), 0 none.
@@ -1245,14 +1248,15 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   // dn: the descriptor of p + 2nw (in flight)
   int4 d0 = uni(load_desc(p));
   int4 d1 = uni(load_desc(p + nw));
-  int4 dn = load_desc(p + 2 * nw);
-  auto step = [&](PieceIn<T> &cur, PieceIn<T> &nxt, PieceIn<T> &far) {
+  int4 d2 = uni(load_desc(p + 2 * nw));
+  int4 dn = load_desc(p + 3 * nw);
+  auto step = [&](PieceIn<T> &cur, PieceIn<T> &nxt, PieceIn<T> &mid, PieceIn<T> &far) {
     const int pc = p;
     const bool more = p + nw < npieces;  // wave-uniform
-    const int4 d2 = uni(dn);
-    piece_load<T>(a, vr, cr, d2, lane, far);   // tiles of p + 2nw
-    dn = load_desc(p + 3 * nw);
+    const int4 d3 = uni(dn);
+    dn = load_desc(p + 4 * nw);
     piece_xload<T, XL>(a, xl, xr, d1, lane, nxt);  // x of p + nw
+    piece_load<T>(a, vr, cr, d3, lane, far);   // tiles of p + 3nw
     const int I = d0.x, jl = d0.y;
     const bool last = (jl & 1) != 0;
     uint32_t t[T];
@@ -1293,15 +1297,17 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
     }
     d0 = d1;
     d1 = d2;
+    d2 = d3;
     p += nw;
     return more;
   };
-  PieceIn<T> b0, b1, b2;
+  PieceIn<T> b0, b1, b2, b3;
   piece_load<T>(a, vr, cr, d0, lane, b0);
   piece_load<T>(a, vr, cr, d1, lane, b1);
+  piece_load<T>(a, vr, cr, d2, lane, b2);
   short_rows();
   piece_xload<T, XL>(a, xl, xr, d0, lane, b0);
-  while (step(b0, b1, b2) && step(b1, b2, b0) && step(b2, b0, b1)) {
+  while (step(b0, b1, b2, b3) && step(b1, b2, b3, b0) && step(b2, b3, b0, b1) && step(b3, b0, b1, b2)) {
   }
 }
 
