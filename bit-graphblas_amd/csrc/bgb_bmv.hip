@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <vector>
 #include <climits>
+#include <cmath>
 #include <type_traits>
 #include <map>
 #include <mutex>
@@ -921,9 +922,19 @@ __device__ __forceinline__ int row_pieces(int s, int e) {
 }
 
 template <int T>
-__global__ void piece_count_kernel(const int *__restrict__ rowptr, int nbr, int *__restrict__ np) {
+__global__ __launch_bounds__(256) void piece_count_kernel(const int *__restrict__ rowptr, int nbr,
+                                                          int *__restrict__ np,
+                                                          unsigned long long *__restrict__ long_tiles) {
   const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I < nbr) np[I] = row_pieces<T>(rowptr[I], rowptr[I + 1]);
+  unsigned long long lt = 0;  // tiles of the rows cut into pieces
+  if (I < nbr) {
+    const int s = rowptr[I], e = rowptr[I + 1], n = row_pieces<T>(s, e);
+    np[I] = n;
+    lt = n > 0 ? (unsigned long long)(e - s) : 0ull;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) lt += __shfl_xor(lt, o);
+  if ((threadIdx.x & 63) == 0 && lt != 0) atomicAdd(long_tiles, lt);
 }
 
 // Piece descriptors: {row, 2 * (piece of the row) + (last piece of the row),
@@ -1181,7 +1192,7 @@ __device__ __forceinline__ void short_row_counts(const BmvArgs &a, const uint8_t
 // memory), 0 none.
 template <int T, int OP, int XL>
 __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes, const int4 *__restrict__ pieces,
-                                                         int npieces, const int *__restrict__ np) {
+                                                         int npieces, const int *__restrict__ np, int ns) {
   static_assert(T == 4 || T == 8, "one lane per tile");
   using W = typename Tile<T>::W;
   using Acc = typename AccOf<OP>::type;
@@ -1199,8 +1210,14 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
-  const int nw = (int)gridDim.x * ((int)blockDim.x >> 6);
-  int p = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6));
+  // The first ns waves of each workgroup count the short rows, the others
+  // walk the pieces: the short rows' dependent loads (row extents, then
+  // tiles) run beside the piece pipeline instead of delaying it.
+  const int wv = (int)threadIdx.x >> 6, wpg = (int)blockDim.x >> 6;
+  const bool srw = wv < ns;
+  const int nw = (int)gridDim.x * (wpg - ns);
+  const int nsw = (int)gridDim.x * ns;
+  int p = __builtin_amdgcn_readfirstlane(srw ? (int)blockIdx.x * ns + wv : (int)blockIdx.x * (wpg - ns) + wv - ns);
   const int p0 = p;
   const __amdgpu_buffer_rsrc_t cr =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(a.colind), (short)0, a.nblocks * 4, 0x00020000);
@@ -1209,12 +1226,11 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.B), (short)0,
                                                                       a.x_words * (int)sizeof(W), 0x00020000);
   // The short rows (np[I] == 0), one lane each, in blocks of 64 consecutive
-  // block rows, wave w taking blocks w, w + nw, ...: done by the same waves
-  // before their pieces, with x read as the pieces read it (from LDS where
-  // staged), while the wave's first piece loads are in flight. Empty rows
-  // get zeros when C is overwritten (the fix-up zeroes none on this path).
+  // block rows, short-row wave w taking blocks w, w + nsw, ..., with x read
+  // as the pieces read it (from LDS where staged). Empty rows get zeros when
+  // C is overwritten (the fix-up zeroes none on this path).
   auto short_rows = [&]() {
-    for (long long blk = p0; blk * 64 < a.nbr; blk += nw) {
+    for (long long blk = p0; blk * 64 < a.nbr; blk += nsw) {
       const long long I = blk * 64 + lane;
       if (I < a.nbr && np[I] == 0) {
         const int s = a.rowptr[I], e = a.rowptr[I + 1];
@@ -1228,10 +1244,11 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
       }
     }
   };
-  if (p >= npieces) {  // uniform per wave; no barrier follows
+  if (srw) {  // uniform per wave; no barrier follows
     short_rows();
     return;
   }
+  if (p >= npieces) return;
   // descriptors as vector loads; past the wave's last piece every load
   // (descriptor, tiles, x) repeats its first piece, results unused, so that
   // all loads stay unconditional and the waits count them exactly
@@ -1305,7 +1322,6 @@ __global__ __launch_bounds__(1024) void bmv_piece_kernel(BmvArgs a, int x_bytes,
   piece_load<T>(a, vr, cr, d0, lane, b0);
   piece_load<T>(a, vr, cr, d1, lane, b1);
   piece_load<T>(a, vr, cr, d2, lane, b2);
-  short_rows();
   piece_xload<T, XL>(a, xl, xr, d0, lane, b0);
   while (step(b0, b1, b2, b3) && step(b1, b2, b3, b0) && step(b2, b3, b0, b1) && step(b3, b0, b1, b2)) {
   }
@@ -1414,6 +1430,7 @@ struct PieceEntry {
   int *np;        // nbr + 1 ints
   int4 *pieces;   // npieces
   int npieces;
+  long long long_tiles;  // tiles of the rows cut into pieces (the others: short rows)
 };
 constexpr int kPieceCache = 8;
 std::mutex g_piece_mu;
@@ -1443,7 +1460,7 @@ int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntr
       }
     }
   }
-  PieceEntry ne{dev, rowptr, nbr, nblocks, T, nullptr, nullptr, 0};
+  PieceEntry ne{dev, rowptr, nbr, nblocks, T, nullptr, nullptr, 0, 0};
   DeviceBuffer off;
   hipError_t he = hipMalloc(&ne.np, sizeof(int) * ((size_t)nbr + 1));
   if (he != hipSuccess) return set_error(BGB_ERR_ALLOC, "piece plan hipMalloc failed: %s", hipGetErrorString(he));
@@ -1453,17 +1470,25 @@ int piece_plan(hipStream_t s, const int *rowptr, int nbr, int nblocks, PieceEntr
     if (ne.pieces) (void)hipFree(ne.pieces);
     return set_error(code, "piece plan: %s: %s", what, hipGetErrorString(err));
   };
-  if ((he = off.alloc(sizeof(int) * ((size_t)nbr + 1), s)) != hipSuccess) return fail(BGB_ERR_ALLOC, "alloc", he);
+  // offsets (nbr + 1 ints), then the long-row tile counter (8 bytes, aligned)
+  const size_t lt_off = (sizeof(int) * ((size_t)nbr + 1) + 7) & ~(size_t)7;
+  if ((he = off.alloc(lt_off + sizeof(unsigned long long), s)) != hipSuccess)
+    return fail(BGB_ERR_ALLOC, "alloc", he);
+  unsigned long long *d_lt = reinterpret_cast<unsigned long long *>(static_cast<char *>(off.ptr) + lt_off);
   fill_kernel<int><<<1, 64, 0, s>>>(ne.np + nbr, 1, 0);
-  piece_count_kernel<T><<<(unsigned)ceil_div(nbr, 256), 256, 0, s>>>(rowptr, nbr, ne.np);
+  fill_kernel<unsigned long long><<<1, 64, 0, s>>>(d_lt, 1, 0ull);
+  piece_count_kernel<T><<<(unsigned)ceil_div(nbr, 256), 256, 0, s>>>(rowptr, nbr, ne.np, d_lt);
   if ((he = hipGetLastError()) != hipSuccess) return fail(BGB_ERR_HIP, "count", he);
   if ((he = exclusive_scan_i32(ne.np, off.as<int>(), (size_t)nbr + 1, s)) != hipSuccess)
     return fail(BGB_ERR_HIP, "scan", he);
   int total = 0;
+  unsigned long long lt = 0;
   if ((he = hipMemcpyAsync(&total, off.as<int>() + nbr, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (he = hipMemcpyAsync(&lt, d_lt, sizeof(lt), hipMemcpyDeviceToHost, s)) != hipSuccess ||
       (he = hipStreamSynchronize(s)) != hipSuccess)
     return fail(BGB_ERR_HIP, "read-back", he);
   ne.npieces = total;
+  ne.long_tiles = (long long)lt;
   if ((he = hipMalloc(&ne.pieces, sizeof(int4) * (size_t)std::max(total, 1))) != hipSuccess)
     return fail(BGB_ERR_ALLOC, "hipMalloc", he);
   if (total > 0) {
@@ -1596,7 +1621,7 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe, const PieceX &
         static_cast<const uint8_t *>(a.B), a.xl_words, a.x_words, bm, px.bm_dwords);
     BGB_HIP_TRY(hipGetLastError());
   }
-  void (*k)(BmvArgs, int, const int4 *, int, const int *) = px.xl == 1   ? &bmv_piece_kernel<T, OP, 1>
+  void (*k)(BmvArgs, int, const int4 *, int, const int *, int) = px.xl == 1 ? &bmv_piece_kernel<T, OP, 1>
                                                           : px.xl == 2 ? &bmv_piece_kernel<T, OP, 2>
                                                           : px.xl == 3 ? &bmv_piece_kernel<T, OP, 3>
                                                                        : &bmv_piece_kernel<T, OP, 0>;
@@ -1606,13 +1631,19 @@ int launch_pieces(BmvArgs a, hipStream_t s, const PieceEntry &pe, const PieceX &
   int per_cu = 0;
   BGB_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), 1024,
                                                            (size_t)px.lds));
-  // waves: one per 16 pieces or per 16 blocks of 64 short-row candidates,
-  // at most the occupancy
+  // short-row waves per workgroup of 16: with the short rows' share of the
+  // tiles (a short-row tile costs a lane's dependent loads, about twice a
+  // piece slot), at least 2 -- R-MAT 20 and 22 at T = 4: 2 of 16 (4: 56.9 /
+  // 332 us, 2: 56.9 / 326 us, 6: 61 / 350 us)
+  const double short_frac = a.nblocks ? (double)(a.nblocks - pe.long_tiles) / (double)a.nblocks : 1.0;
+  const int ns = std::max(2, std::min(8, (int)std::ceil(32.0 * short_frac)));
+  // workgroups: one per 16 - ns pieces or per ns blocks of 64 short-row
+  // candidates, at most the occupancy
   const unsigned grid = (unsigned)std::min<long long>(
       (long long)f.cus * std::max(per_cu, 1),
-      std::max<long long>(1, std::max(ceil_div(pe.npieces, 16), ceil_div(ceil_div(a.nbr, 64), 16))));
+      std::max<long long>(1, std::max(ceil_div(pe.npieces, 16 - ns), ceil_div(ceil_div(a.nbr, 64), ns))));
   const int stage_bytes = px.xl == 0 ? 0 : (int)(px.stage_words * (long long)sizeof(W));
-  k<<<grid, 1024, (size_t)px.lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces, pe.np);
+  k<<<grid, 1024, (size_t)px.lds, s>>>(a, stage_bytes, pe.pieces, pe.npieces, pe.np, ns);
   BGB_HIP_TRY(hipGetLastError());
   return BGB_OK;
 }
@@ -1642,9 +1673,10 @@ int launch_bmv_s(BmvArgs a, hipStream_t s, uint8_t *scr) {
       bool want = g_row_pieces == 2 || (g_x_staging[ilog2_tiledim(T) - 2] != 1 && (px.xl == 1 || px.xl == 3));
       if (want) {
         if ((st = piece_plan<T>(s, a.rowptr, a.nbr, a.nblocks, &pe))) return st;
-        // tiles per piece slot (short rows' tiles included: a bound)
-        const double fill = (double)a.nblocks / ((double)std::max(pe.npieces, 1) * piece_tiles<T>());
-        pieces = g_row_pieces == 2 || fill >= 0.6;
+        // tiles of the long rows per piece slot, and the short rows' share
+        const double fill = (double)pe.long_tiles / ((double)std::max(pe.npieces, 1) * piece_tiles<T>());
+        const double short_frac = a.nblocks ? 1.0 - (double)pe.long_tiles / (double)a.nblocks : 1.0;
+        pieces = g_row_pieces == 2 || (fill >= 0.6 && short_frac <= 0.25);
       }
     }
   }
