@@ -198,6 +198,9 @@ def test_tuning_knobs_validate_and_round_trip():
     assert L.bgb_set_x_staging(6, 0) == 2        # BGB_ERR_TILEDIM
     assert L.bgb_get_short_row_path() == 0
     assert L.bgb_set_short_row_path(3) == 1      # BGB_ERR_ARG
+    assert L.bgb_get_row_pieces() == 0
+    assert L.bgb_set_row_pieces(3) == 1          # BGB_ERR_ARG
+    assert L.bgb_set_row_pieces(-1) == 1
     try:
         assert L.bgb_set_x_staging(16, 2) == 0 and L.bgb_get_x_staging(16) == 2
         assert L.bgb_set_run_length(16, 16) == 0 and L.bgb_get_run_length(16) == 16
