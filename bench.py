@@ -118,13 +118,19 @@ def time_bmv(torch, fn, steps, warmup):
     return s.elapsed_time(e) / 1e3 / steps, wall
 
 
-def time_cold(torch, fn, flush, reps=10):
-    """Median per-call time (s) with a write of `flush` (larger than the 256 MB
-    MALL) before every call, so the call starts with its inputs in HBM only."""
+def time_cold(torch, fn, flush, reps=10, read=False):
+    """Median per-call time (s) with a pass over `flush` (larger than the 256 MB
+    MALL) before every call, so the call starts with its inputs in HBM only:
+    a write (the caches then hold dirty lines, written back while the call
+    runs) or, with read=True, a read (clean lines, dropped as the call's own
+    lines come in)."""
     import numpy as np
     ts = []
     for _ in range(reps):
-        flush.fill_(1)
+        if read:
+            flush.view(torch.int64).sum()
+        else:
+            flush.fill_(1)
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
         fn()
@@ -638,9 +644,11 @@ def main():
         gteps = csr.nnz / t / 1e9
         gbs = nbytes / t / 1e9
         tcold = time_cold(torch, fn, flush)
+        tcold_r = time_cold(torch, fn, flush, read=True)
         per_tile[T] = dict(ms=t * 1e3, gteps=gteps, alg_gbs=gbs, hbm_frac=gbs / HBM_PEAK_GBS,
                            nblocks=A.nblocks, alg_bytes=nbytes, pack_s=round(pack_s, 3),
-                           cold_ms=tcold * 1e3, cold_hbm_frac=nbytes / tcold / 1e9 / HBM_PEAK_GBS)
+                           cold_ms=tcold * 1e3, cold_hbm_frac=nbytes / tcold / 1e9 / HBM_PEAK_GBS,
+                           cold_read_ms=tcold_r * 1e3, cold_read_hbm_frac=nbytes / tcold_r / 1e9 / HBM_PEAK_GBS)
         results[T] = C[: csr.nrows].cpu().numpy()
         log(f"[bench] T={T} nblocks={A.nblocks} {t * 1e3:.4f} ms  {gteps:.1f} GTEPS  "
             f"{gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  pack {pack_s:.2f}s  "
@@ -706,12 +714,15 @@ def main():
                 fn = lambda: B.bmv_bin_bin_full(A, xb, out=C)  # noqa: E731
                 t, _ = time_bmv(torch, fn, args.steps, args.warmup)
                 tcold = time_cold(torch, fn, flush)
+                tcold_r = time_cold(torch, fn, flush, read=True)
                 nbytes = bmv_bytes(T, A.nblockrows, A.nblocks)
                 pt22[str(T)] = {"ms": round(t * 1e3, 4), "gteps": round(csr22.nnz / t / 1e9, 2),
                                 "alg_gbs": round(nbytes / t / 1e9, 1),
                                 "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "nblocks": A.nblocks,
                                 "cold_ms": round(tcold * 1e3, 4), "cold_gteps": round(csr22.nnz / tcold / 1e9, 2),
-                                "cold_hbm_frac": round(nbytes / tcold / 1e9 / HBM_PEAK_GBS, 4)}
+                                "cold_hbm_frac": round(nbytes / tcold / 1e9 / HBM_PEAK_GBS, 4),
+                                "cold_read_ms": round(tcold_r * 1e3, 4),
+                                "cold_read_hbm_frac": round(nbytes / tcold_r / 1e9 / HBM_PEAK_GBS, 4)}
                 log(f"[bench] rmat22 T={T}: {pt22[str(T)]}")
                 del A, xb, C
                 torch.cuda.empty_cache()
@@ -756,8 +767,12 @@ def main():
                      "traffic": measured_traffic(args.config, best),
                      "alg_bytes": b["alg_bytes"],
                      # the same call with a 512 MB write before it (inputs in HBM only,
-                     # none in the MALL): achieved is back-to-back calls on resident data
-                     "cold_frac": round(b["cold_hbm_frac"], 4), "cold_ms": round(b["cold_ms"], 5)},
+                     # none in the MALL; the write's dirty lines are written back during
+                     # the call), and with a 512 MB read before it (clean caches): achieved
+                     # is back-to-back calls on resident data
+                     "cold_frac": round(b["cold_hbm_frac"], 4), "cold_ms": round(b["cold_ms"], 5),
+                     "cold_read_frac": round(b["cold_read_hbm_frac"], 4),
+                     "cold_read_ms": round(b["cold_read_ms"], 5)},
         "cpu_baseline": cpu,
         "copy_ceiling_gbs": round(ceiling, 1),
         "read_ceiling_gbs": round(rceil, 1) if rceil else None,
