@@ -1,6 +1,7 @@
 """Seeded randomized parity against the oracle: every BMV operator across the
 kernel paths a matrix can take (chunked walk with general, one-row and
-two-row chunks; short-row kernel; packed x in LDS or gathered from memory),
+two-row chunks; short-row kernel; row pieces; packed x in LDS or gathered
+from memory),
 the batched BFS, and the K=16 MFMA path single and batched.
 
 Each case draws a graph shape (hub rows spanning many chunks, runs of empty
@@ -52,16 +53,19 @@ def test_random_paths(orc, cuda, seed):
     import bgb.b2sr  # noqa: F401
     from bgb import b2sr as B
     rng = np.random.default_rng(1000 + seed)
+    prng = np.random.default_rng(5000 + seed)  # row-piece mode (a stream of its own)
     try:
         for case in range(6):
             label, csr = random_graph(rng)
             T = int(rng.choice(TS))
             staging = int(rng.integers(0, 3))
             short = int(rng.integers(0, 3)) if T <= 8 else 0
+            pieces = int(prng.integers(0, 3))
             bgb.call("bgb_set_x_staging", T, staging)
             bgb.call("bgb_set_short_row_path", short)
+            bgb.call("bgb_set_row_pieces", pieces)
             A, xb, xfp = check_all_ops(orc, csr, T, rng, cuda,
-                                       f"seed {seed}.{case} {label} staging={staging} short={short}",
+                                       f"seed {seed}.{case} {label} staging={staging} short={short} pieces={pieces}",
                                        xdensity=float(rng.uniform(0.02, 0.9)))
             once = B.bmv_bin_bin_full(A, xb)
             twice = B.bmv_bin_bin_full(A, xb, out=once.clone(), accumulate=True)
@@ -71,6 +75,7 @@ def test_random_paths(orc, cuda, seed):
         for T in TS:
             bgb.call("bgb_set_x_staging", T, 0)
         bgb.call("bgb_set_short_row_path", 0)
+        bgb.call("bgb_set_row_pieces", 0)
 
 
 @pytest.mark.timeout(600)
