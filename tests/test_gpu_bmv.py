@@ -305,6 +305,36 @@ def test_row_pieces_modes(orc, dev, T):
         bgb.call("bgb_set_row_pieces", prev)
 
 
+def test_row_piece_plans_evicted_and_remade(orc, dev):
+    """Piece plans are cached per (device, rowptr, NBR, nblocks, T), eight at a
+    time: ten matrices in turn (T = 4 and 8, pieces forced) evict the oldest
+    plans, and the second pass over all of them remakes those and reuses the
+    rest, every result equal to the oracle's counts."""
+    import torch
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(123)
+    mats = []
+    for i in range(10):
+        T = 4 if i % 2 == 0 else 8
+        csr = graph.rmat(11 + i % 3, 8 + i, seed=200 + i)
+        x01 = (rng.random(csr.nrows) < 0.3).astype(np.float32)
+        A = B.B2SR.from_csr(csr, T, dev)
+        xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+        mats.append((csr, A, xb, orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))))
+    prev = bgb.lib().bgb_get_row_pieces()
+    try:
+        bgb.call("bgb_set_row_pieces", 2)
+        for rnd in range(2):
+            for i, (csr, A, xb, want) in enumerate(mats):
+                log(f"[case] piece plan cache round {rnd} matrix {i} T={A.T}")
+                got = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(A.nblockrows * A.T, torch.int32, dev)))
+                np.testing.assert_array_equal(got[: csr.nrows], want)
+    finally:
+        bgb.call("bgb_set_row_pieces", prev)
+
+
 @pytest.mark.parametrize("T", (4, 8))
 def test_row_pieces_bitmap_x(orc, dev, T):
     """A packed x larger than LDS on the row-piece path: LDS holds the first
