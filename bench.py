@@ -706,6 +706,10 @@ def main():
         if "rmat22" in wanted:
             x22 = graph.bernoulli(csr22.nrows, 0.05, 2)
             pt22 = {}
+            want22 = None
+            if not args.no_cpu:  # the oracle's counts, to check every tile dim's result
+                import oracle_lib
+                want22 = oracle_lib.Oracle().csr_count(csr22.rowptr, csr22.colind, x22.astype(np.uint8))
             flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
             for T in tiles:
                 A = B.B2SR.from_csr(csr22, T, dev)
@@ -723,6 +727,9 @@ def main():
                                 "cold_hbm_frac": round(nbytes / tcold / 1e9 / HBM_PEAK_GBS, 4),
                                 "cold_read_ms": round(tcold_r * 1e3, 4),
                                 "cold_read_hbm_frac": round(nbytes / tcold_r / 1e9 / HBM_PEAK_GBS, 4)}
+                if want22 is not None:
+                    pt22[str(T)]["bit_exact_vs_oracle"] = bool(
+                        np.array_equal(C[: csr22.nrows].cpu().numpy(), want22))
                 log(f"[bench] rmat22 T={T}: {pt22[str(T)]}")
                 del A, xb, C
                 torch.cuda.empty_cache()
@@ -781,6 +788,23 @@ def main():
                               for kk, vv in v.items()} for k, v in per_tile.items()},
     }
     line.update(extras)
+    # the north-star figures in one place: R-MAT 22 bin_bin_full (fastest tile
+    # dim, and the one nearest the HBM roofline) and the batched BFS
+    ns = {}
+    if "bin_bin_full_rmat22" in extras:
+        pt = extras["bin_bin_full_rmat22"]["per_tile"]
+        fast = max(pt, key=lambda k: pt[k]["gteps"])
+        near = max(pt, key=lambda k: pt[k]["hbm_frac"])
+        pick = lambda k: {kk: pt[k].get(kk) for kk in ("ms", "gteps", "hbm_frac", "cold_read_ms",  # noqa: E731
+                                                     "cold_read_hbm_frac", "bit_exact_vs_oracle")}
+        ns["bin_bin_full_rmat22_fastest"] = dict(tile_dim=int(fast), **pick(fast))
+        ns["bin_bin_full_rmat22_nearest_roofline"] = dict(tile_dim=int(near), **pick(near))
+    if "bfs_rmat22" in extras:
+        b = extras["bfs_rmat22"]
+        ns["bfs_rmat22"] = {k: b.get(k) for k in ("tile_dim", "batched_ms", "batched_gteps", "plain_ms", "speedup",
+                                                  "identical_levels")}
+    if ns:
+        line["north_star"] = ns
     print(json.dumps(line), flush=True)
 
 
