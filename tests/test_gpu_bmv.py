@@ -305,6 +305,56 @@ def test_row_pieces_modes(orc, dev, T):
         bgb.call("bgb_set_row_pieces", prev)
 
 
+@pytest.mark.timeout(600)
+def test_row_pieces_x_placements(orc, dev):
+    """The piece kernel's other packed-x placements, pieces forced: x read
+    from memory (x staging off, and a packed x at an address that is not
+    16-byte aligned), and its first words staged with the rest read from
+    memory (T = 4, 6 M vertices: x of 1.5 M words is too large even for the
+    non-zero bitmap). Counts against the oracle, dirty output and accumulate."""
+    import torch
+    import bgb
+    from bgb import b2sr as B
+    from bgb import graph
+    rng = np.random.default_rng(140)
+
+    def check(csr, T, label, misalign=False):
+        log(f"[case] x placement: {label} T={T}")
+        A = B.B2SR.from_csr(csr, T, dev)
+        x01 = (rng.random(csr.ncols) < 0.05).astype(np.float32)
+        xb = B.pack_vec(B.padded_vector(x01, T, dev), T)
+        if misalign:  # the same words one byte into a larger buffer
+            buf = torch.zeros(xb.numel() + 16, dtype=torch.uint8, device=dev)
+            buf[1 : 1 + xb.numel()] = xb
+            xb = buf[1 : 1 + xb.numel()]
+        want = orc.csr_count(csr.rowptr, csr.colind, x01.astype(np.uint8))
+        got = to_np(B.bmv_bin_bin_full(A, xb, out=dirty(A.nblockrows * T, torch.int32, dev)))
+        np.testing.assert_array_equal(got[: csr.nrows], want)
+        twice = B.bmv_bin_bin_full(A, xb, out=torch.from_numpy(got).to(dev), accumulate=True)
+        np.testing.assert_array_equal(to_np(twice)[: csr.nrows], 2 * want)
+
+    prev = bgb.lib().bgb_get_row_pieces()
+    try:
+        bgb.call("bgb_set_row_pieces", 2)
+        small = graph.rmat(16, 16, seed=41)
+        for T in (4, 8):
+            bgb.call("bgb_set_x_staging", T, 1)
+            check(small, T, "staging off (x from memory)")
+            bgb.call("bgb_set_x_staging", T, 0)
+            check(small, T, "x not 16-byte aligned (x from memory)", misalign=True)
+        n, m = 6000000, 6000000
+        src = rng.integers(0, n, m)
+        dst = np.where(rng.random(m) < 0.6, rng.integers(0, n // 8, m), rng.integers(0, n, m))
+        hubs = np.repeat(np.array([3, n // 3, n - 5]), 40000)
+        big = graph.from_coo(n, n, np.concatenate([src, hubs]).astype(np.int32),
+                             np.concatenate([dst, rng.integers(0, n, hubs.size)]).astype(np.int32), False, True)
+        check(big, 4, "first words staged, the rest from memory (6 M vertices)")
+    finally:
+        bgb.call("bgb_set_row_pieces", prev)
+        for T in (4, 8):
+            bgb.call("bgb_set_x_staging", T, 0)
+
+
 def test_row_piece_plans_evicted_and_remade(orc, dev):
     """Piece plans are cached per (device, rowptr, NBR, nblocks, T), eight at a
     time: ten matrices in turn (T = 4 and 8, pieces forced) evict the oldest
