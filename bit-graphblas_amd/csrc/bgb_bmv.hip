@@ -897,18 +897,19 @@ __global__ __launch_bounds__(256) void bmv_rows_kernel(BmvArgs a) {
 // fixed chunks regardless of rows, and pays for it per chunk: the group-row
 // pass, a row-end test per tile, the segmented scan and carries. Here the
 // work is cut along rows instead, classified once per matrix:
-//   * short block rows (at most kShortRowMax tiles): one lane per row, by the
-//     piece kernel's waves before their pieces (x read as the pieces read it);
+//   * short block rows (at most kShortRowMax tiles): one lane per row, by a
+//     few waves of each piece-kernel workgroup (x read as the pieces read it);
 //   * longer rows: pieces of piece_tiles<T>() tiles on a grid aligned to the row's
-//     first 16-byte unit of tile words, one wave per piece: lanes stream
-//     whole 16-byte units (4 tiles at T = 4, 2 at T = 8), count with
-//     byte-wise popcounts, and one DPP wave sum gives the piece's row totals
-//     -- no row ends inside a piece. A row's last piece writes the row (plain
-//     store or accumulate); its other pieces leave carry records, which the
-//     chunk path's fix-up adds (records of one row are consecutive).
-// Tiles outside the piece (the aligned unit's head, loads past the matrix)
-// count nothing: colind and the tile words are read through buffer
-// resources, which return zeros out of range.
+//     first 16-byte unit of tile words, one wave per piece (the workgroup's
+//     other waves): lanes stream whole 16-byte units (4 tiles at T = 4, 2 at
+//     T = 8), count with byte permutes and v_bcnt, and one DPP wave sum per
+//     two rows gives the piece's row totals -- no row ends inside a piece. A
+//     row's last piece writes the row (plain store or accumulate); its other
+//     pieces leave carry records, which the chunk path's fix-up adds (records
+//     of one row are consecutive).
+// Slots outside the piece (the aligned unit's head, the tail of the row's
+// last unit) count nothing (their x words are zeroed); units past the
+// matrix read zeros through the buffer resources' range checks.
 // Tiles per piece: 256 at T = 4 (one 16-byte unit per lane), 256 at T = 8
 // (two units per lane).
 template <int T>
