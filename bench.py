@@ -58,6 +58,17 @@ def measured_traffic(workload, T):
     return e["bytes_per_call"] if e else None
 
 
+def measured_traffic_kernel(workload, T):
+    """The main kernel the committed traffic figure for (workload, T) was
+    measured on, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            e = json.load(f).get(workload, {}).get(str(T))
+    except (OSError, ValueError):
+        return None
+    return e.get("kernel") if e else None
+
+
 def bmv_bytes(T, nbr, nblocks):
     wb = {4: 1, 8: 1, 16: 2, 32: 4, 64: 8}[T]
     x = -(-nbr // T) * T * wb
@@ -772,6 +783,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(b["alg_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4),
                      "traffic": measured_traffic(args.config, best),
+                     # where that figure comes from (rocprofv3 PMC runs of `bench.py --profile`)
+                     "traffic_source": {"file": "profiles/bmv_traffic.json",
+                                        "kernel": measured_traffic_kernel(args.config, best)},
                      "alg_bytes": b["alg_bytes"],
                      # the same call with a 512 MB write before it (inputs in HBM only,
                      # none in the MALL; the write's dirty lines are written back during
