@@ -44,29 +44,44 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "bmv_traffic.json")
 
 
+BMV_SOURCE = os.path.join(REPO, "bit-graphblas_amd", "csrc", "bgb_bmv.hip")
+
+
+def _traffic_entry(workload, T):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(workload, {}).get(str(T))
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic(workload, T):
     """HBM bytes per BMV call at tile dim T for `workload`, from the committed
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --profile`
     (main kernel + fix-up, each counter scaled by what it reads for a 256 MiB
     copy in the same run), or None when not measured."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    e = t.get(workload, {}).get(str(T))
+    e = _traffic_entry(workload, T)
     return e["bytes_per_call"] if e else None
 
 
-def measured_traffic_kernel(workload, T):
-    """The main kernel the committed traffic figure for (workload, T) was
-    measured on, or None."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            e = json.load(f).get(workload, {}).get(str(T))
-    except (OSError, ValueError):
+def traffic_source(workload, T):
+    """Where the traffic figure comes from: the file, the main kernel it was
+    measured on, and whether the BMV kernel source it was measured with is
+    the one in this tree (sha256 of bgb_bmv.hip; None when the entry has no
+    hash), so that a figure left stale by a kernel change says so."""
+    e = _traffic_entry(workload, T)
+    if not e:
         return None
-    return e.get("kernel") if e else None
+    want = e.get("source_sha256")
+    current = None
+    if want:
+        import hashlib
+        try:
+            with open(BMV_SOURCE, "rb") as f:
+                current = hashlib.sha256(f.read()).hexdigest() == want
+        except OSError:
+            current = None
+    return {"file": "profiles/bmv_traffic.json", "kernel": e.get("kernel"), "measured_on_this_source": current}
 
 
 def bmv_bytes(T, nbr, nblocks):
@@ -784,8 +799,7 @@ def main():
                      "unit": "GB/s", "frac": round(b["hbm_frac"], 4),
                      "traffic": measured_traffic(args.config, best),
                      # where that figure comes from (rocprofv3 PMC runs of `bench.py --profile`)
-                     "traffic_source": {"file": "profiles/bmv_traffic.json",
-                                        "kernel": measured_traffic_kernel(args.config, best)},
+                     "traffic_source": traffic_source(args.config, best),
                      "alg_bytes": b["alg_bytes"],
                      # the same call with a 512 MB write before it (inputs in HBM only,
                      # none in the MALL; the write's dirty lines are written back during
